@@ -1,0 +1,77 @@
+// Host-side helpers shared by the graph utilities: a counter-based RNG whose
+// streams do not depend on the thread count, and a small parallel-for over
+// std::thread.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+namespace dfq {
+
+// SplitMix64 (Steele, Lea, Flood 2014): used both as a seeding hash and as
+// the sequential generator of one stream.
+inline uint64_t splitmix64_next(uint64_t &state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline uint64_t hash64(uint64_t x) {
+  uint64_t s = x;
+  return splitmix64_next(s);
+}
+
+// One independent stream per (seed, stream id): edge i of a generator uses
+// stream i, so the output is the same for any partition over threads.
+struct Rng {
+  uint64_t state;
+  Rng(uint64_t seed, uint64_t stream) {
+    state = hash64(seed ^ hash64(stream + 0x632BE59BD9B4E019ull));
+  }
+  uint64_t next() { return splitmix64_next(state); }
+  // Uniform integer in [0, bound) by Lemire's multiply-shift with rejection.
+  uint64_t below(uint64_t bound) {
+    if (bound <= 1) return 0;
+    for (;;) {
+      uint64_t x = next();
+      __uint128_t p = (__uint128_t)x * bound;
+      uint64_t lo = (uint64_t)p;
+      if (lo >= bound) return (uint64_t)(p >> 64);
+      uint64_t thresh = (0 - bound) % bound;
+      if (lo >= thresh) return (uint64_t)(p >> 64);
+    }
+  }
+  // Uniform double in [0, 1) with 53 random bits.
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+inline unsigned host_threads() {
+  unsigned t = std::thread::hardware_concurrency();
+  if (t == 0) t = 1;
+  return std::min(t, 64u);
+}
+
+// Runs fn(begin, end, tid) over [0, n) split into contiguous chunks.
+template <class F>
+void parallel_for(uint64_t n, F fn, unsigned threads = 0) {
+  if (threads == 0) threads = host_threads();
+  if (n < 4096 || threads == 1) {
+    fn((uint64_t)0, n, 0u);
+    return;
+  }
+  threads = (unsigned)std::min<uint64_t>(threads, n);
+  std::vector<std::thread> pool;
+  pool.reserve(threads);
+  uint64_t chunk = (n + threads - 1) / threads;
+  for (unsigned t = 0; t < threads; ++t) {
+    uint64_t b = (uint64_t)t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    pool.emplace_back([=, &fn] { fn(b, e, t); });
+  }
+  for (auto &th : pool) th.join();
+}
+
+}  // namespace dfq

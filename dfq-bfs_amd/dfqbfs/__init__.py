@@ -1,0 +1,357 @@
+"""ctypes view of the MI355X DFQ-BFS C ABI (include/dfq_graph.h, include/dfq_bfs.h).
+
+The product is the native library ``lib/libdfqbfs.so`` (C++ host code and
+HIP kernels).  This module only binds its entry points for the tests and the
+benchmark; it does no BFS arithmetic of its own and has no fallback: if the
+library is missing, importing this module raises.
+
+Names follow the reference host driver
+(design/shared-q/atomic-sa/our-sep/upper-bound/bfs.cu:181-317): a graph is
+loaded or generated on the host (``Csr``), made resident on the GPU
+(``DeviceGraph``), and searched from a source; the result is the per-vertex
+level array (``UNREACHED`` for vertices the search does not reach) and,
+optionally, a BFS-tree parent array.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdfqbfs.so")
+CLI_PATH = os.path.join(PKG_ROOT, "bin", "bfs")
+
+UNREACHED = 0xFFFFFFFF
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_IO = -2
+ERR_FORMAT = -3
+ERR_RANGE = -4
+ERR_NOMEM = -5
+ERR_HIP = -6
+ERR_NO_DEVICE = -7
+ERR_COOPERATIVE = -8
+ERR_CAPACITY = -9
+ERR_BUSY = -10
+ERR_STATE = -11
+
+CSR_SYMMETRIZE = 0x1
+CSR_DEDUP = 0x2
+CSR_DROP_SELF = 0x4
+CSR_PERMUTE = 0x8
+CSR_UNDIRECTED = CSR_SYMMETRIZE | CSR_DEDUP | CSR_DROP_SELF | CSR_PERMUTE
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `make -C {PKG_ROOT}` (or __graft_entry__.build())")
+
+lib = C.CDLL(LIB_PATH)
+
+
+class DfqError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        super().__init__(f"{what}: {lib.dfq_strerror(code).decode()} ({code})")
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint64),
+                ("row_offsets", C.POINTER(C.c_int64)), ("col_idx", C.POINTER(C.c_uint32))]
+
+
+class Config(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("small_max", C.c_uint32), ("medium_max", C.c_uint32),
+                ("large_max", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("depth", C.c_uint32), ("grid_blocks", C.c_uint32),
+                ("block_size", C.c_uint32), ("reserved0", C.c_uint32), ("reached", C.c_uint64)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("device", C.c_int32), ("cu_count", C.c_int32), ("xcc_count", C.c_int32),
+                ("cooperative", C.c_int32), ("lds_per_cu", C.c_uint64), ("hbm_bytes", C.c_uint64),
+                ("l2_bytes", C.c_uint64), ("arch", C.c_char * 64)]
+
+
+class ValidateReport(C.Structure):
+    _fields_ = [("errors", C.c_uint64), ("bad_source", C.c_uint64), ("bad_edge_span", C.c_uint64),
+                ("missed_vertex", C.c_uint64), ("bad_parent", C.c_uint64), ("reached", C.c_uint64),
+                ("depth", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = C.c_void_p
+_u32p = C.POINTER(C.c_uint32)
+_i64p = C.POINTER(C.c_int64)
+_csrp = C.POINTER(_Csr)
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("dfq_strerror", C.c_char_p, C.c_int)
+_sig("dfq_csr_free", None, _csrp)
+_sig("dfq_csr_from_edges", C.c_int, _u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
+     C.POINTER(_csrp))
+_sig("dfq_csr_from_arrays", C.c_int, _i64p, _u32p, C.c_uint32, C.c_uint64, C.POINTER(_csrp))
+_sig("dfq_load_snap", C.c_int, C.c_char_p, C.c_uint32, C.POINTER(_csrp))
+_sig("dfq_save_snap", C.c_int, C.c_char_p, _csrp)
+_sig("dfq_load_dat", C.c_int, C.c_char_p, C.POINTER(_csrp), _u32p)
+_sig("dfq_save_dat", C.c_int, C.c_char_p, _csrp, C.c_uint32)
+_sig("dfq_load_csr_bin", C.c_int, C.c_char_p, C.POINTER(_csrp))
+_sig("dfq_save_csr_bin", C.c_int, C.c_char_p, _csrp)
+_sig("dfq_gen_amazon_like", C.c_int, C.c_uint32, C.c_uint64, C.POINTER(_csrp))
+_sig("dfq_gen_rmat", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+     C.c_uint64, C.c_uint32, C.POINTER(_csrp))
+_sig("dfq_gen_grid2d", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(_csrp))
+_sig("dfq_pick_sources", C.c_int, _csrp, C.c_uint32, C.c_uint64, _u32p)
+_sig("dfq_reached_work", C.c_int, _csrp, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+     _u32p)
+_sig("dfq_validate_bfs", C.c_int, _csrp, C.c_uint32, _u32p, _u32p, C.POINTER(ValidateReport))
+_sig("dfq_compare_levels", C.c_int, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint64))
+_sig("dfq_config_default", C.c_int, C.POINTER(Config))
+_sig("dfq_get_device_info", C.c_int, C.POINTER(DeviceInfo))
+_sig("dfq_graph_create", C.c_int, _i64p, _u32p, C.c_uint32, C.c_uint64, C.POINTER(_P))
+_sig("dfq_graph_destroy", None, _P)
+_sig("dfq_graph_set_config", C.c_int, _P, C.POINTER(Config))
+_sig("dfq_graph_get_config", C.c_int, _P, C.POINTER(Config))
+_sig("dfq_graph_device_arrays", C.c_int, _P, C.POINTER(_P), C.POINTER(_P))
+_sig("dfq_bfs", C.c_int, _P, C.c_uint32, _u32p, _u32p, C.POINTER(Stats))
+_sig("dfq_bfs_device", C.c_int, _P, C.c_uint32, _P, _P, C.POINTER(Stats))
+
+
+def check(rc: int, what: str):
+    if rc != OK:
+        raise DfqError(rc, what)
+
+
+def _u32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _i64(a):
+    return a.ctypes.data_as(_i64p)
+
+
+class Csr:
+    """Host CSR owned by the native library (freed with the object)."""
+
+    def __init__(self, ptr):
+        self._ptr = ptr
+
+    @property
+    def n(self) -> int:
+        return self._ptr.contents.n
+
+    @property
+    def m(self) -> int:
+        return self._ptr.contents.m
+
+    @property
+    def row_offsets(self) -> np.ndarray:
+        c = self._ptr.contents
+        return np.ctypeslib.as_array(c.row_offsets, shape=(c.n + 1,))
+
+    @property
+    def col_idx(self) -> np.ndarray:
+        c = self._ptr.contents
+        if c.m == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(c.col_idx, shape=(c.m,))
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row_offsets)
+
+    def __del__(self):
+        if getattr(self, "_ptr", None):
+            lib.dfq_csr_free(self._ptr)
+            self._ptr = None
+
+    # -- constructors ------------------------------------------------------
+    @staticmethod
+    def _new(fn, what, *args) -> "Csr":
+        out = _csrp()
+        check(fn(*args, C.byref(out)), what)
+        return Csr(out)
+
+    @classmethod
+    def from_edges(cls, src, dst, n: int, flags: int = 0, perm_seed: int = 0) -> "Csr":
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        if src.shape != dst.shape:
+            raise ValueError("src and dst differ in length")
+        return cls._new(lib.dfq_csr_from_edges, "dfq_csr_from_edges", _u32(src), _u32(dst),
+                        src.size, n, flags, perm_seed)
+
+    @classmethod
+    def from_arrays(cls, row_offsets, col_idx) -> "Csr":
+        ro = np.ascontiguousarray(row_offsets, np.int64)
+        ci = np.ascontiguousarray(col_idx, np.uint32)
+        return cls._new(lib.dfq_csr_from_arrays, "dfq_csr_from_arrays", _i64(ro), _u32(ci),
+                        ro.size - 1, ci.size)
+
+    @classmethod
+    def rmat(cls, scale: int, edgefactor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1,
+             flags: int = CSR_UNDIRECTED) -> "Csr":
+        return cls._new(lib.dfq_gen_rmat, "dfq_gen_rmat", scale, edgefactor, a, b, c, seed, flags)
+
+    @classmethod
+    def amazon_like(cls, n: int = 262144, seed: int = 0xD0F1) -> "Csr":
+        return cls._new(lib.dfq_gen_amazon_like, "dfq_gen_amazon_like", n, seed)
+
+    @classmethod
+    def grid2d(cls, width: int, height: int, drop_p: float = 0.0, seed: int = 4096) -> "Csr":
+        return cls._new(lib.dfq_gen_grid2d, "dfq_gen_grid2d", width, height, drop_p, seed)
+
+    @classmethod
+    def load_snap(cls, path: str, flags: int = 0) -> "Csr":
+        return cls._new(lib.dfq_load_snap, "dfq_load_snap", path.encode(), flags)
+
+    @classmethod
+    def load_csr_bin(cls, path: str) -> "Csr":
+        return cls._new(lib.dfq_load_csr_bin, "dfq_load_csr_bin", path.encode())
+
+    @classmethod
+    def load_dat(cls, path: str):
+        out = _csrp()
+        src = C.c_uint32(0)
+        check(lib.dfq_load_dat(path.encode(), C.byref(out), C.byref(src)), "dfq_load_dat")
+        return Csr(out), src.value
+
+    # -- writers / queries -------------------------------------------------
+    def save_snap(self, path: str):
+        check(lib.dfq_save_snap(path.encode(), self._ptr), "dfq_save_snap")
+
+    def save_dat(self, path: str, source: int = 0):
+        check(lib.dfq_save_dat(path.encode(), self._ptr, source), "dfq_save_dat")
+
+    def save_csr_bin(self, path: str):
+        check(lib.dfq_save_csr_bin(path.encode(), self._ptr), "dfq_save_csr_bin")
+
+    def pick_sources(self, count: int, seed: int) -> np.ndarray:
+        out = np.zeros(max(count, 1), np.uint32)
+        check(lib.dfq_pick_sources(self._ptr, count, seed, _u32(out)), "dfq_pick_sources")
+        return out[:count]
+
+    def reached_work(self, level: np.ndarray):
+        """(directed entries of reached vertices, reached vertices, depth)."""
+        level = np.ascontiguousarray(level, np.uint32)
+        e, v, d = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        check(lib.dfq_reached_work(self._ptr, _u32(level), C.byref(e), C.byref(v), C.byref(d)),
+              "dfq_reached_work")
+        return e.value, v.value, d.value
+
+    def validate(self, source: int, level: np.ndarray, parent: np.ndarray | None = None) -> dict:
+        level = np.ascontiguousarray(level, np.uint32)
+        pp = None
+        if parent is not None:
+            parent = np.ascontiguousarray(parent, np.uint32)
+            pp = _u32(parent)
+        rep = ValidateReport()
+        check(lib.dfq_validate_bfs(self._ptr, source, _u32(level), pp, C.byref(rep)),
+              "dfq_validate_bfs")
+        return rep.as_dict()
+
+
+def default_config() -> Config:
+    c = Config()
+    check(lib.dfq_config_default(C.byref(c)), "dfq_config_default")
+    return c
+
+
+def device_info() -> dict:
+    info = DeviceInfo()
+    check(lib.dfq_get_device_info(C.byref(info)), "dfq_get_device_info")
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    d["arch"] = info.arch.decode()
+    return d
+
+
+@dataclass
+class BfsResult:
+    level: np.ndarray
+    parent: np.ndarray | None
+    kernel_ms: float
+    depth: int
+    reached: int
+    grid_blocks: int
+    block_size: int
+
+
+class DeviceGraph:
+    """A CSR resident in GPU memory with its BFS working set."""
+
+    def __init__(self, csr: Csr | None = None, *, row_offsets=None, col_idx=None):
+        if csr is not None:
+            ro, ci, n, m = csr._ptr.contents.row_offsets, csr._ptr.contents.col_idx, csr.n, csr.m
+        else:
+            self._ro = np.ascontiguousarray(row_offsets, np.int64)
+            self._ci = np.ascontiguousarray(col_idx, np.uint32)
+            ro, ci, n, m = _i64(self._ro), _u32(self._ci), self._ro.size - 1, self._ci.size
+        self.n, self.m = n, m
+        h = _P()
+        check(lib.dfq_graph_create(ro, ci, n, m, C.byref(h)), "dfq_graph_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.dfq_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def config(self) -> Config:
+        c = Config()
+        check(lib.dfq_graph_get_config(self._h, C.byref(c)), "dfq_graph_get_config")
+        return c
+
+    def set_config(self, cfg: Config | None = None, **kw):
+        c = cfg if cfg is not None else self.config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        check(lib.dfq_graph_set_config(self._h, C.byref(c)), "dfq_graph_set_config")
+
+    def bfs(self, source: int, parents: bool = False) -> BfsResult:
+        level = np.empty(self.n, np.uint32)
+        parent = np.empty(self.n, np.uint32) if parents else None
+        st = Stats()
+        check(lib.dfq_bfs(self._h, source, _u32(level), _u32(parent) if parents else None,
+                          C.byref(st)), "dfq_bfs")
+        return BfsResult(level, parent, st.kernel_ms, st.depth, st.reached, st.grid_blocks,
+                         st.block_size)
+
+    def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
+        """BFS into caller-owned device buffers (raw device addresses)."""
+        st = Stats()
+        check(lib.dfq_bfs_device(self._h, source, C.c_void_p(d_level),
+                                 C.c_void_p(d_parent) if d_parent else None, C.byref(st)),
+              "dfq_bfs_device")
+        return st

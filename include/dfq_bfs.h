@@ -1,0 +1,113 @@
+/*
+ * dfq_bfs.h — device BFS engine of the MI355X DFQ-BFS library (plain C ABI).
+ *
+ * Replaces the reference's device path of
+ * design/shared-q/atomic-sa/our-sep/upper-bound/bfs.cu:
+ *
+ *   reference                                            replaced by
+ *   ---------------------------------------------------  ---------------------------
+ *   :186,194-197 cudaMallocManaged node/edge/cost        dfq_graph_create (explicit
+ *   :273-277     cudaMemPrefetchAsync                     device copies, no managed
+ *                                                         memory)
+ *   :98-179      __global__ CUDA_BFS_KERNEL_CG_NEW(Node*, dfq_bfs / dfq_bfs_device:
+ *                int*, short*, bool*, int*)              one persistent cooperative
+ *   :255-271     occupancy query, grid = SMs x blocks     launch per BFS on gfx950,
+ *   :279-288     cudaEvent timing around                  hipEvent timing in
+ *                cudaLaunchCooperativeKernel + sync       dfq_stats.kernel_ms
+ *   :312-316     cudaFree                                dfq_graph_destroy
+ *
+ * Output convention: level[v] is the hop distance from the source (uint32,
+ * DFQ_UNREACHED when unreachable; the reference used short cost[] with -1).
+ * parent[v] (optional) is a BFS-tree parent: parent[source] == source,
+ * DFQ_UNREACHED for unreached vertices.
+ *
+ * All calls return DFQ_OK (0) or a negative DFQ_ERR_* code (dfq_graph.h).
+ * A dfq_graph may be used by one host thread at a time.
+ */
+#ifndef DFQ_BFS_H
+#define DFQ_BFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dfq_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tuning knobs of the persistent kernel (the reference fixed these with
+ * #defines, bfs.cu:16-36 upper-bound, lower-bound/bfs.cu:16-41).
+ * Degree bins: deg <= small_max       -> lane-owned, wave-gathered edges
+ *              deg <= medium_max      -> one wavefront (64 lanes) per vertex
+ *              deg <= large_max       -> one workgroup per vertex
+ *              deg >  large_max       -> whole grid, HBM queue                */
+typedef struct dfq_config {
+  uint32_t block_size;     /* threads per workgroup: 256 or 512           */
+  uint32_t blocks_per_cu;  /* 0 = as many as occupancy allows             */
+  uint32_t small_max;      /* <= 64                                        */
+  uint32_t medium_max;     /* >= small_max                                 */
+  uint32_t large_max;      /* >= medium_max                                */
+  uint32_t reserved[3];    /* must be zero                                 */
+} dfq_config;
+
+typedef struct dfq_stats {
+  double   kernel_ms;      /* hipEvent time of the cooperative launch     */
+  uint32_t depth;          /* number of BFS levels (max level + 1)        */
+  uint32_t grid_blocks;    /* workgroups launched                         */
+  uint32_t block_size;     /* threads per workgroup                       */
+  uint32_t reserved0;
+  uint64_t reached;        /* vertices with a level                       */
+} dfq_stats;
+
+typedef struct dfq_device_info {
+  int32_t  device;              /* HIP ordinal                            */
+  int32_t  cu_count;            /* compute units                          */
+  int32_t  xcc_count;           /* accelerator complex dies (0 = unknown) */
+  int32_t  cooperative;         /* cooperative launch supported           */
+  uint64_t lds_per_cu;          /* bytes of LDS per CU                    */
+  uint64_t hbm_bytes;           /* global memory                          */
+  uint64_t l2_bytes;            /* L2 per XCD as reported                 */
+  char     arch[64];            /* gcnArchName                            */
+} dfq_device_info;
+
+typedef struct dfq_graph dfq_graph;   /* opaque, device-resident CSR */
+
+/* Defaults used when no config is set (tuned for gfx950). */
+int dfq_config_default(dfq_config *cfg);
+
+/* Information about the current HIP device; DFQ_ERR_NO_DEVICE without one. */
+int dfq_get_device_info(dfq_device_info *info);
+
+/* Upload a host CSR (row_offsets[n+1] int64, col_idx[m] uint32) to HBM and
+ * allocate the BFS working set (bitmap, queues, control block).  Row offsets
+ * must be non-decreasing with row_offsets[0] == 0 and row_offsets[n] == m,
+ * and every column index < n; violations give DFQ_ERR_RANGE. */
+int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx,
+                     uint32_t n, uint64_t m, dfq_graph **out);
+void dfq_graph_destroy(dfq_graph *g);
+
+int dfq_graph_set_config(dfq_graph *g, const dfq_config *cfg);
+int dfq_graph_get_config(const dfq_graph *g, dfq_config *cfg);
+
+/* Device pointers of the resident CSR (for callers that keep other device
+ * data next to it).  Valid until dfq_graph_destroy. */
+int dfq_graph_device_arrays(const dfq_graph *g, const int64_t **d_row_offsets,
+                            const uint32_t **d_col_idx);
+
+/* Single-source BFS.  level_out[n] is required, parent_out[n] optional
+ * (NULL skips parent output); both are host arrays.  stats optional.
+ * Blocks until the result is in the host arrays. */
+int dfq_bfs(dfq_graph *g, uint32_t source, uint32_t *level_out,
+            uint32_t *parent_out, dfq_stats *stats);
+
+/* Same BFS with outputs left in device memory (d_level[n] required,
+ * d_parent[n] optional).  Blocks until the kernel has finished; the timed
+ * region of stats->kernel_ms is the cooperative launch only. */
+int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
+                   uint32_t *d_parent, dfq_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFQ_BFS_H */

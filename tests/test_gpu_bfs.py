@@ -1,0 +1,180 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle and the
+golden vectors: levels bit-exact for every vertex, parent tree valid."""
+import numpy as np
+import pytest
+
+from conftest import UNREACHED, csr_from_edges_np, golden_names, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def check_result(dfq, csr, source, res, expect):
+    np.testing.assert_array_equal(res.level, expect, err_msg=f"source {source}")
+    if res.parent is not None:
+        rep = csr.validate(source, res.level, res.parent)
+        assert rep["errors"] == 0, rep
+    reached = int((expect != UNREACHED).sum())
+    assert res.reached == reached
+    assert res.depth == int(expect[expect != UNREACHED].max()) + 1
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden(dfq, oracle, name):
+    gd = load_golden(name)
+    csr = dfq.Csr.from_arrays(gd["row_offsets"], gd["col_idx"])
+    with dfq.DeviceGraph(csr) as g:
+        for s, expect in zip(gd["sources"], gd["levels"]):
+            say("golden", name, "source", int(s))
+            res = g.bfs(int(s), parents=True)
+            check_result(dfq, csr, int(s), res, expect)
+            np.testing.assert_array_equal(expect, oracle.bfs(gd["row_offsets"], gd["col_idx"], int(s)))
+
+
+CONFIGS = [
+    dict(block_size=256, small_max=32, medium_max=2048, large_max=32768),
+    dict(block_size=512, small_max=32, medium_max=2048, large_max=32768),
+    dict(block_size=256, small_max=1, medium_max=1, large_max=1),      # everything huge
+    dict(block_size=256, small_max=64, medium_max=64, large_max=64),   # no wave/WG bins
+    dict(block_size=256, small_max=0, medium_max=4096, large_max=4096),
+    dict(block_size=512, small_max=8, medium_max=64, large_max=1024),
+    dict(block_size=256, small_max=16, medium_max=256, large_max=16384, blocks_per_cu=1),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(CONFIGS)))
+def test_bin_configs_on_hubs(dfq, oracle, ci):
+    gd = load_golden("bin_hubs_50000")
+    csr = dfq.Csr.from_arrays(gd["row_offsets"], gd["col_idx"])
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(**CONFIGS[ci])
+        for s, expect in zip(gd["sources"], gd["levels"]):
+            say("config", CONFIGS[ci], "source", int(s))
+            check_result(dfq, csr, int(s), g.bfs(int(s), parents=True), expect)
+
+
+def test_closed_forms(dfq):
+    # long directed path: one vertex per level (2000 levels, 2000 barriers)
+    n = 2000
+    ro, ci = csr_from_edges_np(n, np.arange(n - 1), np.arange(1, n))
+    csr = dfq.Csr.from_arrays(ro, ci)
+    with dfq.DeviceGraph(csr) as g:
+        say("path 2000")
+        r = g.bfs(0, parents=True)
+        np.testing.assert_array_equal(r.level, np.arange(n, dtype=np.uint32))
+        np.testing.assert_array_equal(r.parent[1:], np.arange(n - 1, dtype=np.uint32))
+        r = g.bfs(n - 1)
+        assert r.level[n - 1] == 0 and (r.level[:-1] == UNREACHED).all()
+    # star with a 300k-degree hub: whole-grid expansion of a single vertex
+    n = 300001
+    ro, ci = csr_from_edges_np(n, np.zeros(n - 1, int), np.arange(1, n))
+    csr = dfq.Csr.from_arrays(ro, ci)
+    with dfq.DeviceGraph(csr) as g:
+        say("star 300k")
+        r = g.bfs(0, parents=True)
+        assert r.level[0] == 0 and (r.level[1:] == 1).all() and (r.parent[1:] == 0).all()
+    # complete binary tree
+    n = (1 << 16) - 1
+    child = np.arange(1, n)
+    ro, ci = csr_from_edges_np(n, (child - 1) // 2, child)
+    csr = dfq.Csr.from_arrays(ro, ci)
+    with dfq.DeviceGraph(csr) as g:
+        say("binary tree")
+        r = g.bfs(0, parents=True)
+        np.testing.assert_array_equal(r.level, np.floor(np.log2(np.arange(n) + 1)).astype(np.uint32))
+        assert csr.validate(0, r.level, r.parent)["errors"] == 0
+
+
+def test_single_vertex_and_isolated_source(dfq):
+    csr = dfq.Csr.from_arrays([0, 0], np.zeros(0, np.uint32))
+    with dfq.DeviceGraph(csr) as g:
+        r = g.bfs(0, parents=True)
+        assert r.level.tolist() == [0] and r.parent.tolist() == [0] and r.depth == 1
+    ro, ci = csr_from_edges_np(5, [1, 2], [2, 3])
+    csr = dfq.Csr.from_arrays(ro, ci)
+    with dfq.DeviceGraph(csr) as g:
+        r = g.bfs(4)
+        assert r.level.tolist() == [UNREACHED] * 4 + [0]
+        with pytest.raises(dfq.DfqError) as e:
+            g.bfs(5)
+        assert e.value.code == dfq.ERR_RANGE
+
+
+@pytest.mark.parametrize("scale", [10, 14, 17])
+def test_rmat_many_sources(dfq, oracle, scale):
+    csr = dfq.Csr.rmat(scale, 16, seed=scale)
+    srcs = csr.pick_sources(8, seed=7)
+    with dfq.DeviceGraph(csr) as g:
+        for s in srcs:
+            say("rmat", scale, "source", int(s))
+            check_result(dfq, csr, int(s), g.bfs(int(s), parents=True),
+                         oracle.bfs(csr.row_offsets, csr.col_idx, int(s)))
+
+
+def test_amazon_like_config1(dfq, oracle):
+    """Config 1 graph (262,144 vertices, directed), source 0."""
+    csr = dfq.Csr.amazon_like(262144, seed=0xD0F1)
+    with dfq.DeviceGraph(csr) as g:
+        say("amazon-like source 0")
+        check_result(dfq, csr, 0, g.bfs(0, parents=True), oracle.bfs(csr.row_offsets, csr.col_idx, 0))
+
+
+def test_rmat22_full_size(dfq, oracle):
+    """Config 2 graph at full size: RMAT scale 22, edgefactor 16."""
+    csr = dfq.Csr.rmat(22, 16, seed=22)
+    srcs = csr.pick_sources(64, seed=2022)
+    with dfq.DeviceGraph(csr) as g:
+        for i, s in enumerate(srcs[:4]):
+            say("rmat22 source", int(s))
+            check_result(dfq, csr, int(s), g.bfs(int(s), parents=(i % 2 == 0)),
+                         oracle.bfs(csr.row_offsets, csr.col_idx, int(s)))
+        # the rest: size-independent properties (certificate + reach count)
+        e_ref = None
+        for s in srcs[4:16]:
+            say("rmat22 certificate source", int(s))
+            r = g.bfs(int(s), parents=True)
+            assert csr.validate(int(s), r.level, r.parent)["errors"] == 0
+            e, v, _ = csr.reached_work(r.level)
+            e_ref = e_ref or e
+            assert e == e_ref  # every source lies in the one giant component
+
+
+def test_grid_config3(dfq, oracle):
+    """Config 3 graph: 4096 x 4096 grid, 2% edges dropped, source (0,0)."""
+    csr = dfq.Csr.grid2d(4096, 4096, 0.02, seed=4096)
+    with dfq.DeviceGraph(csr) as g:
+        say("grid 4096")
+        r = g.bfs(0, parents=True)
+        check_result(dfq, csr, 0, r, oracle.bfs(csr.row_offsets, csr.col_idx, 0))
+        assert r.depth > 8000
+
+
+def test_repeat_runs_share_no_state(dfq, oracle):
+    """Back-to-back searches on one resident graph: nothing left over."""
+    csr = dfq.Csr.rmat(16, 8, seed=3)
+    srcs = csr.pick_sources(6, seed=1)
+    with dfq.DeviceGraph(csr) as g:
+        expect = {int(s): oracle.bfs(csr.row_offsets, csr.col_idx, int(s)) for s in srcs}
+        for rep in range(3):
+            for s in srcs:
+                r = g.bfs(int(s))
+                np.testing.assert_array_equal(r.level, expect[int(s)], err_msg=f"rep {rep} src {s}")
+
+
+def test_device_outputs(dfq, oracle):
+    torch = pytest.importorskip("torch")
+    csr = dfq.Csr.rmat(15, 16, seed=9)
+    s = int(csr.pick_sources(1, 3)[0])
+    lv = torch.empty(csr.n, dtype=torch.int32, device="cuda")
+    par = torch.empty(csr.n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    with dfq.DeviceGraph(csr) as g:
+        st = g.bfs_device(s, lv.data_ptr(), par.data_ptr())
+        assert st.kernel_ms > 0
+    level = lv.cpu().numpy().view(np.uint32)
+    parent = par.cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(level, oracle.bfs(csr.row_offsets, csr.col_idx, s))
+    assert csr.validate(s, level, parent)["errors"] == 0

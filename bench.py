@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X DFQ-BFS engine on one GPU.
+
+Workload (the headline configuration): undirected RMAT graph, scale 22
+(4,194,304 vertices), edgefactor 16 (67,108,864 generated edges),
+a/b/c/d = 0.57/0.19/0.19/0.05, self loops and duplicates removed,
+symmetrised, vertex ids permuted, uint32 columns with int64 row offsets;
+64 seeded sources drawn from vertices with degree > 0.
+
+A step is one BFS from each of the 64 sources (one persistent cooperative
+launch per source, outputs written to device buffers that stay resident).
+`value` is the harmonic mean over the sources of the per-source GTEPS,
+GTEPS = (E_r / 2) / t_kernel with E_r the directed CSR entries of the
+reached component (Graph500 convention for undirected graphs) and t_kernel
+the hipEvent time of the launch on its stream, averaged over the timed
+steps.  The roofline entry prices the algorithmic bytes of one search,
+B_alg = 4 E_r + 16 V_r + 4 N (CSR entries, two int64 offsets per reached
+vertex, the level array written in full), against 8 TB/s.
+
+cpu_baseline times the oracle (serial CPU BFS restating the reference's
+host loop) on a bounded sample of the same sources and also checks that the
+GPU levels equal the oracle's bit for bit on that sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dfq-bfs_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+MODES = {
+    # every level top-down through the degree-binned frontier queues
+    "topdown": dict(block_size=1024, small_max=32, medium_max=2048, large_max=32768,
+                    direction_opt=0, bu_enter_div=14, bu_exit_div=24),
+    # the same kernel with bottom-up levels for large frontiers
+    "hybrid": dict(block_size=1024, small_max=32, medium_max=2048, large_max=32768,
+                   direction_opt=1, bu_enter_div=256, bu_exit_div=1024),
+}
+
+
+def run_mode(dfq, g, srcs, work, steps, warmup, d_level):
+    """Per-source mean kernel ms over the timed steps, and wall ms per step."""
+    for _ in range(warmup):
+        for s in srcs:
+            g.bfs_device(int(s), d_level.ptr)
+    dfq.synchronize()
+    times = np.zeros((steps, len(srcs)))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        for i, s in enumerate(srcs):
+            times[k, i] = g.bfs_device(int(s), d_level.ptr).kernel_ms
+    dfq.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    t = times.mean(axis=0)
+    e_r = np.array([w[0] for w in work], dtype=np.float64)
+    v_r = np.array([w[1] for w in work], dtype=np.float64)
+    gteps = (e_r / 2) / (t * 1e-3) / 1e9
+    hmean = len(gteps) / np.sum(1.0 / gteps)
+    b_alg = 4 * e_r + 16 * v_r + 4 * g.n
+    achieved = b_alg.sum() / (t.sum() * 1e-3) / 1e9
+    return dict(gteps_hmean=float(hmean), gteps_median=float(np.median(gteps)),
+                ms_per_bfs_mean=float(t.mean()), ms_per_bfs_median=float(np.median(t)),
+                wall_ms_per_step=wall_ms, achieved_gbs=float(achieved),
+                b_alg_mean=float(b_alg.mean()), kernel_ms_total=float(t.sum()))
+
+
+def cpu_baseline(csr, srcs, levels_gpu, budget_s):
+    """Serial oracle BFS on as many sources as fit in budget_s (at least 2)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle  # the checker; never the measured GPU path
+    orc = Oracle()
+    done, edges, secs, exact = 0, 0, 0.0, True
+    for i, s in enumerate(srcs):
+        t0 = time.perf_counter()
+        lv = orc.bfs(csr.row_offsets, csr.col_idx, int(s))
+        dt = time.perf_counter() - t0
+        e, _, _ = csr.reached_work(lv)
+        edges += e / 2
+        secs += dt
+        done += 1
+        exact &= bool(np.array_equal(lv, levels_gpu[i]))
+        if done >= 2 and secs >= budget_s:
+            break
+    return dict(value=edges / secs / 1e9, unit="GTEPS", cores=1, kind="port",
+                sample=f"{done} of the {len(srcs)} sources, serial oracle BFS (C, -O3), "
+                       f"{secs:.1f} s of CPU time on {platform.processor() or platform.machine()}",
+                levels_bit_exact_on_sample=exact)
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edgefactor", type=int, default=16)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--mode", choices=sorted(MODES), default="hybrid")
+    ap.add_argument("--also", choices=sorted(MODES) + ["none"], default="topdown",
+                    help="second mode reported beside the headline")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the library's streaming copy of BYTES (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the search on one "
+              f"GPU (multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import dfqbfs as dfq
+    info = dfq.device_info()
+    log("device", info)
+    if args.copy_calibration:
+        src, dst = dfq.DeviceBuffer(args.copy_calibration), dfq.DeviceBuffer(args.copy_calibration)
+        ms = [dfq.stream_copy(dst, src, args.copy_calibration) for _ in range(3)]
+        print(json.dumps({"copy_bytes": args.copy_calibration, "launches": 3,
+                          "ms": [round(x, 4) for x in ms],
+                          "GBps_read_plus_write": round(2 * args.copy_calibration / min(ms) / 1e6, 1)}))
+        return
+    t0 = time.perf_counter()
+    csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)
+    log(f"graph n={csr.n} m={csr.m} built in {time.perf_counter() - t0:.1f}s")
+    srcs = csr.pick_sources(args.sources, seed=2022)
+    g = dfq.DeviceGraph(csr)
+    d_level = dfq.DeviceBuffer(4 * csr.n)
+
+    # one checked run per source: the certificate (levels + parent tree) and
+    # the reached work that prices GTEPS and B_alg
+    g.set_config(**MODES[args.mode])
+    work, levels = [], []
+    for s in srcs:
+        r = g.bfs(int(s), parents=True)
+        rep = csr.validate(int(s), r.level, r.parent)
+        if rep["errors"]:
+            raise SystemExit(f"BFS certificate failed for source {s}: {rep}")
+        work.append(csr.reached_work(r.level))
+        levels.append(r.level)
+    log("certificate ok on all sources")
+
+    res = run_mode(dfq, g, srcs, work, args.steps, args.warmup, d_level)
+    log(args.mode, res)
+    extra = {}
+    if args.also != "none" and args.also != args.mode:
+        g.set_config(**MODES[args.also])
+        other = run_mode(dfq, g, srcs, work, args.steps, args.warmup, d_level)
+        log(args.also, other)
+        extra = {args.also: dict(gteps_hmean=round(other["gteps_hmean"], 3),
+                                 ms_per_bfs_mean=round(other["ms_per_bfs_mean"], 4),
+                                 roofline_frac=round(other["achieved_gbs"] / HBM_PEAK_GBS, 4))}
+        g.set_config(**MODES[args.mode])
+
+    traffic = load_traffic(f"traffic_rmat{args.scale}_{args.mode}.json")
+    out = {
+        "metric": "GTEPS + % HBM roofline (RMAT-22)",
+        "value": round(res["gteps_hmean"], 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["wall_ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"RMAT scale {args.scale}, edgefactor {args.edgefactor}, "
+                               f"a/b/c=0.57/0.19/0.19, undirected, {args.sources} sources/step, "
+                               f"levels output",
+                   "vertices": csr.n, "csr_entries": csr.m, "mode": args.mode,
+                   "kernel_config": MODES[args.mode]},
+        "roofline": {"bound": "hbm", "achieved": round(res["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(res["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None},
+        "gteps_median": round(res["gteps_median"], 3),
+        "ms_per_bfs_mean": round(res["ms_per_bfs_mean"], 4),
+        "b_alg_bytes_mean": int(res["b_alg_mean"]),
+    }
+    out.update(extra)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(csr, srcs, levels, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,15 +1,17 @@
 // Device-side building blocks of the DFQ-BFS kernels for gfx950 (CDNA4,
-// wave64): wavefront primitives, the visited-bitmap claim, and the per-wave
-// LDS staging of next-frontier entries.
+// wave64): wavefront primitives, the visited-bitmap claim, the grid barrier
+// and the per-wave output blocks of the decentralised frontier queue.
 //
 // Reference counterparts (design/shared-q/atomic-sa/our-sep/upper-bound/
 // bfs.cu): VISIT_tiled / VISIT_blk_tiled / VISIT_grid_tiled /
 // VISIT_all_tiled (:44-94) test `cost[nid] < 0` and write `cost[ftr] + 1`
-// without atomics, and find the next frontier by rescanning all N levels
-// (:116-124).  Here a vertex is claimed exactly once through an atomic OR on
-// a visited bitmap; the claimer alone writes its level (so levels are
-// deterministic) and parent, and appends it to the next-frontier queue, so no
-// level ever rescans the vertex set.
+// without atomics, find the next frontier by rescanning all N levels
+// (:116-124) and separate levels with cooperative-groups grid.sync()
+// (:114,170,177).  Here a vertex is claimed exactly once through an atomic OR
+// on a visited bitmap; the claimer alone writes its level (so levels are
+// deterministic) and parent and appends it to its wavefront's block of the
+// next-frontier queue; levels are separated by a tree barrier that also sums
+// the next frontier's size.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,8 +44,8 @@ __device__ __forceinline__ uint32_t bcast_lane(uint32_t x, int lane) {
 // Inclusive prefix sum over the 64 lanes of a wavefront.  row_shr DPP for
 // the steps inside a 16-lane row, row_bcast:15 / row_bcast:31 across rows.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-  // update_dpp(old, src, dpp_ctrl, row_mask, bank_mask, bound_ctrl):
-  // lanes whose source is out of the row read `old` (0 here).
+  // update_dpp(old, src, dpp_ctrl, row_mask, bank_mask, bound_ctrl): lanes
+  // whose source is outside the row, or whose row is masked off, get `old`.
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
@@ -53,44 +55,232 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
   return x;
 }
 
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Grid barrier, XCD-aware.
+//
+// MI355X has 8 XCDs, each with its own L2; a store is visible to another XCD
+// only after its L2 line is written back, and a load on another XCD may hit a
+// stale line of that XCD's L2 until it is invalidated.  The language-level
+// agent fence does both per workgroup (buffer_wbl2 / buffer_inv sc1), which
+// at 1024 workgroups costs ~16 us per barrier and, issued by workgroups that
+// arrive early, keeps flushing the L2 under the ones still working.  Here:
+//   * every wave waits for its own stores (s_waitcnt vmcnt(0)): they are then
+//     in its XCD's L2;
+//   * workgroups arrive at their XCD's counter; the last one of each XCD
+//     writes that L2 back once (buffer_wbl2 sc1) and arrives at the root;
+//   * the root's last arrival publishes generation + payload sum in one
+//     64-bit word;
+//   * each XCD's leader invalidates its L2 (buffer_inv sc1) and then
+//     releases its XCD; every other workgroup invalidates again (device
+//     scope is the only one that drops its CU's L1 lines) before reading
+//     shared data.
+// Workgroups find their XCD from the XCC_ID hardware register; the number
+// of workgroups per XCD is counted at kernel start (barrier_census), because
+// the dispatcher's block -> XCD rotation changes from launch to launch.
+// Arrival words pack count (bits 40..63) and payload (bits 0..39).
+// ---------------------------------------------------------------------------
+constexpr int kMaxXcd = 16;
+constexpr uint64_t kArrive = 1ull << 40;
+constexpr uint64_t kPayloadMask = kArrive - 1;
+
+struct alignas(256) BarrierWord {
+  uint64_t word;
+  uint64_t pad[31];
+};
+
+struct GridBarrier {
+  BarrierWord xleaf[kMaxXcd];   // per-XCD arrivals
+  BarrierWord xrel[kMaxXcd];    // per-XCD release generation (low 32 bits)
+  BarrierWord root;             // XCD-leader arrivals
+  BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
+  BarrierWord census_total;     // workgroups counted at kernel start
+  uint32_t census[kMaxXcd];     // workgroups per XCD
+};
+
+struct BarrierState {
+  uint32_t gen;     // generation modulo 2^24
+  uint32_t xcc;     // this workgroup's XCD
+  uint32_t xcc_n;   // workgroups on this XCD
+  uint32_t nx;      // XCDs with workgroups
+};
+
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(v));
+  return v & (kMaxXcd - 1);
+}
+
+__device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void l2_writeback() {
+  asm volatile("buffer_wbl2 sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+}
+__device__ __forceinline__ void l2_invalidate() {
+  asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <class T>
+__device__ __forceinline__ T ld_relaxed(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Kernel start (thread 0 of every workgroup, all workgroups resident):
+// counts workgroups per XCD.  The counters are cleared again by
+// barrier_census_clear after the last barrier of the launch.
+__device__ __forceinline__ void barrier_census(GridBarrier *b, BarrierState &st) {
+  st.xcc = xcc_id();
+  st.gen = (uint32_t)(ld_relaxed(&b->gensum.word) >> 40);
+  // the per-XCD count must land before the total that releases the readers
+  const uint32_t mine = __hip_atomic_fetch_add(&b->census[st.xcc], 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+  wait_own_stores();
+  __hip_atomic_fetch_add(&b->census_total.word, (uint64_t)(mine != 0xffffffffu), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  while (ld_relaxed(&b->census_total.word) < gridDim.x) __builtin_amdgcn_s_sleep(1);
+  st.xcc_n = ld_relaxed(&b->census[st.xcc]);
+  uint32_t nx = 0;
+  for (int i = 0; i < kMaxXcd; ++i) nx += ld_relaxed(&b->census[i]) != 0;
+  st.nx = nx;
+}
+
+__device__ __forceinline__ void barrier_census_clear(GridBarrier *b) {
+  for (int i = 0; i < kMaxXcd; ++i)
+    __hip_atomic_store(&b->census[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&b->census_total.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All threads of every workgroup must call it.  `payload` is read from
+// thread 0; returns the grid-wide payload sum in every thread.  `st` is only
+// meaningful in thread 0.  `s_word` is a 64-bit LDS broadcast slot.
+__device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &st, uint64_t payload,
+                                                 uint64_t *s_word) {
+  wait_own_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t g = st.gen, next = (g + 1) & 0xffffffu;
+    const uint64_t old = __hip_atomic_fetch_add(&b->xleaf[st.xcc].word, kArrive | payload,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t total;
+    if ((old >> 40) + 1 == st.xcc_n) {
+      // XCD leader: every workgroup of this XCD has its stores in this L2
+      const uint64_t xsum = (old & kPayloadMask) + payload;
+      __hip_atomic_store(&b->xleaf[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      l2_writeback();
+      const uint64_t o2 = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if ((o2 >> 40) + 1 == st.nx) {
+        total = (o2 & kPayloadMask) + xsum;
+        __hip_atomic_store(&b->root.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_own_stores();
+        __hip_atomic_store(&b->gensum.word, ((uint64_t)next << 40) | total, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        uint64_t w;
+        while (((w = ld_relaxed(&b->gensum.word)) >> 40) != next) __builtin_amdgcn_s_sleep(1);
+        total = w & kPayloadMask;
+      }
+      l2_invalidate();
+      __hip_atomic_store((uint32_t *)&b->xrel[st.xcc].word, next, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (ld_relaxed((uint32_t *)&b->xrel[st.xcc].word) != next) __builtin_amdgcn_s_sleep(1);
+      total = ld_relaxed(&b->gensum.word) & kPayloadMask;
+      // this CU's L1 may hold stale lines; on gfx950 only the device-scope
+      // invalidate reaches it (workgroup scope leaves L1 alone outside
+      // thread-group-split mode), and it also drops this XCD's L2 lines
+      l2_invalidate();
+    }
+    st.gen = next;
+    *s_word = total;
+  }
+  __syncthreads();
+  return *s_word;
+}
+
+// ---------------------------------------------------------------------------
+// Decentralised frontier queue.  Every wavefront of the grid owns a "home"
+// block of kBlk entries in each queue buffer and, when it fills, takes pool
+// blocks from a per-level cursor (one atomic per kBlk claims).  A block's
+// fill count is published with a plain store when the block fills and at the
+// end of the level.  The consumer side reads block counts after the barrier;
+// no tail counter is shared by the whole grid.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBlk = 256;
+
+struct QueueBuf {
+  uint32_t *ent;   // [(home + pool_cap) * kBlk]
+  uint32_t *cnt;   // [home + pool_cap]
+};
+
 // Visited-bitmap claim: a plain load filters vertices already seen (a stale
 // copy can only say "unvisited", never the reverse, because bits are only
-// set during a BFS); the atomic OR decides ownership.
-__device__ __forceinline__ bool claim_vertex(uint32_t *vis, uint32_t w) {
+// set during a search); the atomic OR decides ownership.
+__device__ __forceinline__ bool claim_vertex(uint32_t *vis, uint32_t w, uint32_t &atomics) {
   const uint32_t word = w >> 5, bit = 1u << (w & 31u);
   if (vis[word] & bit) return false;
+  ++atomics;
   const uint32_t old = __hip_atomic_fetch_or(&vis[word], bit, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
   return (old & bit) == 0;
 }
 
-// Everything the edge visitor needs; one per thread, with `out_n` (entries
-// staged by this wave) kept wave-uniform.
+// Per-thread state of the edge visitor; blk / tail / claims are
+// wave-uniform.
 struct Visitor {
   const uint32_t *__restrict__ col;
   uint32_t *level;
-  uint32_t *parent;   // may be null
+  uint32_t *parent;     // may be null
   uint32_t *vis;
-  uint32_t *qout;     // next-frontier queue (HBM)
-  uint32_t *q_count;  // its tail counter
-  uint32_t *stage;    // this wave's LDS staging buffer
-  uint32_t stage_cap; // entries; a multiple of 64, >= 128
-  uint32_t out_n;     // staged entries (wave-uniform)
+  QueueBuf out;         // next-frontier buffer
+  uint32_t *pool_cursor;
+  uint32_t home_blocks; // index of the first pool block
+  uint32_t pool_cap;    // pool blocks available
+  uint32_t *err;        // set on pool exhaustion (cannot happen when sized as documented)
+  uint32_t blk;         // current output block
+  uint32_t tail;        // entries in it
+  uint32_t claims;      // entries appended this level by this wave
   uint32_t next_level;
+  uint32_t probes, atomics;  // profiling counters (per lane)
 
-  // Reserve a contiguous block of the HBM queue for `cnt` entries and copy
-  // the wave's staged entries there (one atomic per flush).
-  __device__ __forceinline__ void flush_wave() {
-    const uint32_t cnt = out_n;
-    if (cnt == 0) return;
-    uint32_t base = 0;
-    if (lane_id() == 0)
-      base = __hip_atomic_fetch_add(q_count, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    base = bcast_lane(base, 0);
-    wave_sync();
-    for (uint32_t k = lane_id(); k < cnt; k += kWave) qout[base + k] = stage[k];
-    wave_sync();
-    out_n = 0;
+  __device__ __forceinline__ void begin_level(uint32_t home, uint32_t lvl) {
+    blk = home;
+    tail = 0;
+    claims = 0;
+    next_level = lvl;
+  }
+
+  // Publishes the current block's fill count (call at the end of a phase).
+  __device__ __forceinline__ void publish() {
+    if (lane_id() == 0) out.cnt[blk] = tail;
+  }
+
+  // Appends the lanes flagged in `mask` (lane rank r = mbcnt) to the queue.
+  __device__ __forceinline__ void append(bool mine, uint64_t mask, uint32_t w) {
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const uint32_t rank = mbcnt(mask);
+    const uint32_t room = kBlk - tail;
+    if (mine && rank < room) out.ent[(size_t)blk * kBlk + tail + rank] = w;
+    if (cnt < room) {
+      tail += cnt;
+    } else {
+      // current block is full: publish it and continue in a new pool block
+      if (lane_id() == 0) out.cnt[blk] = kBlk;
+      uint32_t j = 0;
+      if (lane_id() == 0)
+        j = __hip_atomic_fetch_add(pool_cursor, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      j = bcast_lane(j, 0);
+      if (j >= pool_cap) {
+        if (lane_id() == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        j = pool_cap - 1;  // keep writes in bounds; the search reports an error
+      }
+      blk = home_blocks + j;
+      tail = cnt - room;
+      if (mine && rank >= room) out.ent[(size_t)blk * kBlk + (rank - room)] = w;
+    }
+    claims += cnt;
   }
 
   // Called by every lane of the wave (wave-uniform control flow); `act`
@@ -98,17 +288,15 @@ struct Visitor {
   __device__ __forceinline__ void visit(bool act, uint32_t u, uint32_t w) {
     bool claimed = false;
     if (act) {
-      claimed = claim_vertex(vis, w);
+      ++probes;
+      claimed = claim_vertex(vis, w, atomics);
       if (claimed) {
         level[w] = next_level;
         if (parent) parent[w] = u;
       }
     }
     const uint64_t mask = __ballot(claimed);
-    if (mask == 0) return;
-    if (claimed) stage[out_n + mbcnt(mask)] = w;
-    out_n += (uint32_t)__popcll(mask);
-    if (out_n > stage_cap - kWave) flush_wave();
+    if (mask) append(claimed, mask, w);
   }
 };
 

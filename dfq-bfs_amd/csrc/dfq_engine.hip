@@ -14,12 +14,29 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "dfq_bfs.h"
 #include "dfq_kernels.hpp"
 #include "host_common.hpp"
 
 using namespace dfq;
+
+namespace {
+
+// Streaming copy used to calibrate traffic counters (16-byte vectors).
+__global__ __launch_bounds__(256) void dfq_stream_copy_kernel(uint4 *__restrict__ dst,
+                                                              const uint4 *__restrict__ src, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+hipError_t &last_hip_error() {
+  static thread_local hipError_t e = hipSuccess;
+  return e;
+}
+
+}  // namespace
 
 #define HIP_TRY(expr)                 \
   do {                                \
@@ -30,20 +47,6 @@ using namespace dfq;
     }                                 \
   } while (0)
 
-namespace {
-
-hipError_t &last_hip_error() {
-  static thread_local hipError_t e = hipSuccess;
-  return e;
-}
-
-struct LaunchShape {
-  int grid = 0;
-  int block = 0;
-};
-
-}  // namespace
-
 struct dfq_graph {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -51,98 +54,183 @@ struct dfq_graph {
   uint32_t n = 0;
   uint64_t m = 0;
   uint32_t n_words = 0;
+  bool symmetric = false;
   int64_t *d_off = nullptr;
   uint32_t *d_col = nullptr;
   uint32_t *d_level = nullptr;   // outputs for the host-array entry point
   uint32_t *d_parent = nullptr;
   uint32_t *d_vis = nullptr;
-  uint32_t *d_q0 = nullptr, *d_q1 = nullptr, *d_hq = nullptr;
+  uint32_t *d_fb = nullptr;
+  uint32_t *d_hq = nullptr;
+  uint32_t *d_qent[2] = {nullptr, nullptr};
+  uint32_t *d_qcnt[2] = {nullptr, nullptr};
+  uint32_t home_blocks = 0, pool_cap = 0;  // queue buffers sized for this grid
+  uint64_t *d_prof = nullptr;
+  uint32_t prof_levels = 0;
   Control *d_ctl = nullptr;
   dfq_config cfg{};
-  LaunchShape shape{};
+  int grid = 0;
+  int block = 0;
 };
 
 namespace {
 
 int validate_config(const dfq_config &c) {
-  if (c.block_size != 256 && c.block_size != 512) return DFQ_ERR_INVALID_ARG;
+  if (c.block_size != 256 && c.block_size != 512 && c.block_size != 1024) return DFQ_ERR_INVALID_ARG;
   if (c.small_max > 64) return DFQ_ERR_INVALID_ARG;
   if (c.medium_max < c.small_max || c.large_max < c.medium_max) return DFQ_ERR_INVALID_ARG;
-  if (c.reserved[0] || c.reserved[1] || c.reserved[2]) return DFQ_ERR_INVALID_ARG;
+  if (c.direction_opt > 1) return DFQ_ERR_INVALID_ARG;
+  if (c.direction_opt && (c.bu_enter_div == 0 || c.bu_exit_div == 0)) return DFQ_ERR_INVALID_ARG;
+  if (c.reserved != 0) return DFQ_ERR_INVALID_ARG;
   return DFQ_OK;
 }
 
 const void *kernel_for(uint32_t block) {
-  return block == 512 ? (const void *)&dfq_bfs_kernel<512> : (const void *)&dfq_bfs_kernel<256>;
+  switch (block) {
+    case 1024: return (const void *)&dfq_bfs_kernel<1024>;
+    case 512: return (const void *)&dfq_bfs_kernel<512>;
+    default: return (const void *)&dfq_bfs_kernel<256>;
+  }
 }
 
-// Grid = CUs x co-resident workgroups per CU (cooperative launch requires
-// every workgroup to be resident).
-int compute_shape(dfq_graph *g) {
+void free_queues(dfq_graph *g) {
+  for (int i = 0; i < 2; ++i) {
+    (void)hipFree(g->d_qent[i]);
+    (void)hipFree(g->d_qcnt[i]);
+    g->d_qent[i] = g->d_qcnt[i] = nullptr;
+  }
+  g->home_blocks = g->pool_cap = 0;
+}
+
+// Grid = CUs x co-resident workgroups per CU (a cooperative launch needs
+// every workgroup resident).  Queue buffers hold one home block per
+// wavefront plus ceil(n / kBlk) + home pool blocks, the most one level can
+// take (every wave's pool blocks are full except its last).
+int configure(dfq_graph *g, const dfq_config &cfg) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   if (!prop.cooperativeLaunch) return DFQ_ERR_COOPERATIVE;
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(g->cfg.block_size),
-                                                        (int)g->cfg.block_size, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(cfg.block_size),
+                                                        (int)cfg.block_size, 0));
   if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
-  if (g->cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)g->cfg.blocks_per_cu);
-  g->shape.grid = prop.multiProcessorCount * per_cu;
-  g->shape.block = (int)g->cfg.block_size;
+  if (cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)cfg.blocks_per_cu);
+  const int grid = prop.multiProcessorCount * per_cu;
+  const uint32_t home = (uint32_t)grid * (cfg.block_size / kWave);
+  const uint32_t pool = (uint32_t)((g->n + kBlk - 1) / kBlk) + home;
+  if (home != g->home_blocks || pool != g->pool_cap) {
+    free_queues(g);
+    const size_t blocks = (size_t)home + pool;
+    for (int i = 0; i < 2; ++i) {
+      HIP_TRY(hipMalloc(&g->d_qent[i], sizeof(uint32_t) * blocks * kBlk));
+      HIP_TRY(hipMalloc(&g->d_qcnt[i], sizeof(uint32_t) * blocks));
+      HIP_TRY(hipMemset(g->d_qcnt[i], 0, sizeof(uint32_t) * blocks));
+    }
+    g->home_blocks = home;
+    g->pool_cap = pool;
+  }
+  g->cfg = cfg;
+  g->grid = grid;
+  g->block = (int)cfg.block_size;
   return DFQ_OK;
 }
 
 void free_device(dfq_graph *g) {
+  free_queues(g);
   (void)hipFree(g->d_off);
   (void)hipFree(g->d_col);
   (void)hipFree(g->d_level);
   (void)hipFree(g->d_parent);
   (void)hipFree(g->d_vis);
-  (void)hipFree(g->d_q0);
-  (void)hipFree(g->d_q1);
+  (void)hipFree(g->d_fb);
   (void)hipFree(g->d_hq);
+  (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
 }
 
-int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats) {
-  if (source >= g->n) return DFQ_ERR_RANGE;
-  HIP_TRY(hipSetDevice(g->device));
+// Bottom-up levels scan a vertex's out-edges as its in-edges, which is only
+// valid when every u->v has a v->u.  Requires sorted rows (binary search);
+// a graph with unsorted rows is treated as directed.
+bool is_symmetric(const int64_t *off, const uint32_t *col, uint32_t n) {
+  std::atomic<bool> ok{true};
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
+      for (int64_t k = off[u] + 1; k < off[u + 1]; ++k)
+        if (col[k - 1] > col[k]) {
+          ok = false;
+          return;
+        }
+  });
+  if (!ok) return false;
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
+      for (int64_t k = off[u]; k < off[u + 1]; ++k) {
+        const uint32_t v = col[k];
+        if (!std::binary_search(col + off[v], col + off[v + 1], (uint32_t)u)) {
+          ok = false;
+          return;
+        }
+      }
+  });
+  return ok;
+}
+
+BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, bool prof) {
   BfsArgs a;
   a.off = g->d_off;
   a.col = g->d_col;
   a.level = d_level;
   a.parent = d_parent;
   a.vis = g->d_vis;
-  a.q[0] = g->d_q0;
-  a.q[1] = g->d_q1;
+  a.fb = g->d_fb;
+  for (int i = 0; i < 2; ++i) {
+    a.q[i].ent = g->d_qent[i];
+    a.q[i].cnt = g->d_qcnt[i];
+  }
   a.hq = g->d_hq;
   a.ctl = g->d_ctl;
+  a.prof = prof ? g->d_prof : nullptr;
+  a.prof_levels = prof ? g->prof_levels : 0;
   a.n = g->n;
   a.n_words = g->n_words;
   a.source = source;
   a.small_max = g->cfg.small_max;
   a.medium_max = g->cfg.medium_max;
   a.large_max = g->cfg.large_max;
+  a.home_blocks = g->home_blocks;
+  a.pool_cap = g->pool_cap;
+  a.bottom_up = (g->cfg.direction_opt && g->symmetric) ? 1u : 0u;
+  a.bu_enter = g->cfg.bu_enter_div ? g->n / g->cfg.bu_enter_div : g->n;
+  a.bu_exit = g->cfg.bu_exit_div ? g->n / g->cfg.bu_exit_div : 0;
+  return a;
+}
+
+int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats,
+            bool prof = false) {
+  if (source >= g->n) return DFQ_ERR_RANGE;
+  HIP_TRY(hipSetDevice(g->device));
+  BfsArgs a = make_args(g, source, d_level, d_parent, prof);
   void *args[] = {&a};
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->shape.grid),
-                                     dim3(g->shape.block), args, 0, g->stream));
+  HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block),
+                                     args, 0, g->stream));
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipEventSynchronize(g->ev1));
+  Control c;
+  HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  if (c.err) return DFQ_ERR_CAPACITY;
   if (stats) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-    Control c;
-    HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
     stats->kernel_ms = ms;
     stats->depth = c.depth;
     stats->reached = c.reached;
-    stats->grid_blocks = (uint32_t)g->shape.grid;
-    stats->block_size = (uint32_t)g->shape.block;
-    stats->reserved0 = 0;
+    stats->grid_blocks = (uint32_t)g->grid;
+    stats->block_size = (uint32_t)g->block;
+    stats->bottom_up_levels = c.bu_levels;
   }
   return DFQ_OK;
 }
@@ -159,6 +247,9 @@ int dfq_config_default(dfq_config *cfg) {
   cfg->small_max = 32;
   cfg->medium_max = 2048;
   cfg->large_max = 32768;
+  cfg->direction_opt = 0;
+  cfg->bu_enter_div = 64;
+  cfg->bu_exit_div = 512;
   return DFQ_OK;
 }
 
@@ -177,8 +268,6 @@ int dfq_get_device_info(dfq_device_info *info) {
   info->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   info->hbm_bytes = prop.totalGlobalMem;
   info->l2_bytes = (uint64_t)prop.l2CacheSize;
-  int xcc = 0;
-  (void)xcc;
   info->xcc_count = 0;
   strncpy(info->arch, prop.gcnArchName, sizeof(info->arch) - 1);
   return DFQ_OK;
@@ -212,7 +301,6 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
 
   dfq_graph *g = new (std::nothrow) dfq_graph();
   if (!g) return DFQ_ERR_NOMEM;
-  int rc = DFQ_OK;
   auto fail = [&](int code) {
     free_device(g);
     delete g;
@@ -230,7 +318,7 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   g->n = n;
   g->m = m;
   g->n_words = (n + 31) / 32;
-  dfq_config_default(&g->cfg);
+  g->symmetric = is_symmetric(row_offsets, col_idx, n);
   HIP_OR_FAIL(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   HIP_OR_FAIL(hipEventCreate(&g->ev0));
   HIP_OR_FAIL(hipEventCreate(&g->ev1));
@@ -239,15 +327,16 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   HIP_OR_FAIL(hipMalloc(&g->d_level, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_parent, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_vis, sizeof(uint32_t) * g->n_words));
-  HIP_OR_FAIL(hipMalloc(&g->d_q0, sizeof(uint32_t) * n));
-  HIP_OR_FAIL(hipMalloc(&g->d_q1, sizeof(uint32_t) * n));
+  HIP_OR_FAIL(hipMalloc(&g->d_fb, sizeof(uint32_t) * ((size_t)g->n_words + 1)));
   HIP_OR_FAIL(hipMalloc(&g->d_hq, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_ctl, sizeof(Control)));
   HIP_OR_FAIL(hipMemcpy(g->d_off, row_offsets, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
   if (m) HIP_OR_FAIL(hipMemcpy(g->d_col, col_idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
   HIP_OR_FAIL(hipMemset(g->d_ctl, 0, sizeof(Control)));
 #undef HIP_OR_FAIL
-  rc = compute_shape(g);
+  dfq_config cfg;
+  dfq_config_default(&cfg);
+  int rc = configure(g, cfg);
   if (rc != DFQ_OK) return fail(rc);
   *out = g;
   return DFQ_OK;
@@ -265,15 +354,9 @@ int dfq_graph_set_config(dfq_graph *g, const dfq_config *cfg) {
   if (!g || !cfg) return DFQ_ERR_INVALID_ARG;
   int rc = validate_config(*cfg);
   if (rc != DFQ_OK) return rc;
-  dfq_config old = g->cfg;
-  LaunchShape old_shape = g->shape;
-  g->cfg = *cfg;
-  rc = compute_shape(g);
-  if (rc != DFQ_OK) {
-    g->cfg = old;
-    g->shape = old_shape;
-  }
-  return rc;
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return configure(g, *cfg);
 }
 
 int dfq_graph_get_config(const dfq_graph *g, dfq_config *cfg) {
@@ -281,6 +364,8 @@ int dfq_graph_get_config(const dfq_graph *g, dfq_config *cfg) {
   *cfg = g->cfg;
   return DFQ_OK;
 }
+
+int dfq_graph_is_symmetric(const dfq_graph *g) { return g && g->symmetric ? 1 : 0; }
 
 int dfq_graph_device_arrays(const dfq_graph *g, const int64_t **d_row_offsets, const uint32_t **d_col_idx) {
   if (!g) return DFQ_ERR_INVALID_ARG;
@@ -301,6 +386,83 @@ int dfq_bfs(dfq_graph *g, uint32_t source, uint32_t *level_out, uint32_t *parent
 int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats) {
   if (!g || !d_level) return DFQ_ERR_INVALID_ARG;
   return run_bfs(g, source, d_level, d_parent, stats);
+}
+
+int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows, uint32_t max_levels, dfq_stats *stats) {
+  if (!g || !rows || max_levels == 0) return DFQ_ERR_INVALID_ARG;
+  HIP_TRY(hipSetDevice(g->device));
+  if (max_levels > g->prof_levels) {
+    (void)hipFree(g->d_prof);
+    g->d_prof = nullptr;
+    g->prof_levels = 0;
+    HIP_TRY(hipMalloc(&g->d_prof, sizeof(uint64_t) * kProfFields * max_levels));
+    g->prof_levels = max_levels;
+  }
+  HIP_TRY(hipMemset(g->d_prof, 0, sizeof(uint64_t) * kProfFields * g->prof_levels));
+  int rc = run_bfs(g, source, g->d_level, nullptr, stats, true);
+  if (rc != DFQ_OK) return rc;
+  HIP_TRY(hipMemcpy(rows, g->d_prof, sizeof(uint64_t) * kProfFields * max_levels, hipMemcpyDeviceToHost));
+  return DFQ_OK;
+}
+
+int dfq_device_alloc(size_t bytes, void **d_ptr) {
+  if (!d_ptr || bytes == 0) return DFQ_ERR_INVALID_ARG;
+  *d_ptr = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DFQ_ERR_NO_DEVICE;
+  HIP_TRY(hipMalloc(d_ptr, bytes));
+  return DFQ_OK;
+}
+
+int dfq_device_free(void *d_ptr) {
+  if (!d_ptr) return DFQ_OK;
+  HIP_TRY(hipFree(d_ptr));
+  return DFQ_OK;
+}
+
+int dfq_copy_to_host(void *h_dst, const void *d_src, size_t bytes) {
+  if (!h_dst || !d_src) return DFQ_ERR_INVALID_ARG;
+  HIP_TRY(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  return DFQ_OK;
+}
+
+int dfq_copy_to_device(void *d_dst, const void *h_src, size_t bytes) {
+  if (!d_dst || !h_src) return DFQ_ERR_INVALID_ARG;
+  HIP_TRY(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return DFQ_OK;
+}
+
+int dfq_stream_copy(void *d_dst, const void *d_src, size_t bytes, double *ms_out) {
+  if (!d_dst || !d_src || bytes == 0 || bytes % 16) return DFQ_ERR_INVALID_ARG;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int dev = 0;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  const size_t n16 = bytes / 16;
+  const size_t want = (n16 + 255) / 256;
+  const unsigned grid = (unsigned)std::min<size_t>(want, (size_t)prop.multiProcessorCount * 8);
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  hipLaunchKernelGGL(dfq_stream_copy_kernel, dim3(grid), dim3(256), 0, nullptr, (uint4 *)d_dst,
+                     (const uint4 *)d_src, n16);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_out) *ms_out = ms;
+  return DFQ_OK;
+}
+
+int dfq_device_synchronize(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DFQ_ERR_NO_DEVICE;
+  HIP_TRY(hipDeviceSynchronize());
+  return DFQ_OK;
 }
 
 }  // extern "C"

@@ -2,11 +2,11 @@
 //
 // One launch runs a whole BFS (reference: CUDA_BFS_KERNEL_CG_NEW,
 // design/shared-q/atomic-sa/our-sep/upper-bound/bfs.cu:98-179, launched
-// cooperatively at :283).  Per level:
+// cooperatively at :283).  Per level (top-down):
 //
-//   phase A  workgroups take chunks of the frontier queue (HBM, written by
-//            the previous level) with an atomic cursor and bin each vertex by
-//            out-degree:
+//   phase A  every workgroup reads its share of the frontier blocks written
+//            by the previous level (round-robin over the per-wave blocks, no
+//            shared cursor) and bins each vertex by out-degree:
 //              deg <= small_max   lane-owned; the wave gathers the edges of
 //                                 its 64 lanes' vertices through a DPP prefix
 //                                 sum so every lane reads one edge per step
@@ -16,19 +16,22 @@
 //            LDS queues are expanded by the workgroup that filled them when
 //            they come within one chunk of full, and drained at the end of
 //            the level (reference :126-169, flush on near-full + drain).
-//   barrier  grid.sync
+//   barrier  tree barrier; returns the number of vertices claimed
 //   phase H  only when the level has huge-degree vertices: all workgroups
 //            split their concatenated edge ranges into workgroup-sized
-//            chunks (reference E/F bins :171-176, which serialised F)
-//            followed by a second grid.sync.
+//            chunks (reference E/F bins :171-176, which serialised F),
+//            then a second barrier.
 //
-// Claimed vertices go to a per-wave LDS staging buffer and from there to the
-// next-frontier queue with one atomic per flush.  The loop ends when the
-// next frontier is empty; every workgroup reads the same counter after the
-// barrier, so all leave on the same level.
+// Optionally (symmetric graphs, direction_opt) a level whose frontier is
+// large runs bottom-up instead: every unvisited vertex looks for a neighbour
+// in the frontier bitmap and stops at the first one (Beamer, Asanovic,
+// Patterson, "Direction-optimizing breadth-first search", SC 2012).  Each
+// vertex has one owner lane, so that level needs no atomics.
+//
+// The loop ends when a level claims no vertex; every workgroup receives the
+// same count from the barrier, so all leave on the same level.
 #pragma once
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,24 +39,35 @@
 
 namespace dfq {
 
-namespace cg = cooperative_groups;
-
-// Per-level control slot; three rotate so that the slot being zeroed for
-// level L+2 is never touched during level L (+ padding against false sharing
-// of the hot counters).
-struct alignas(128) LevelSlot {
-  uint32_t q_count;   // frontier entries of this level (written at level-1)
-  uint32_t q_cursor;  // consumption cursor
-  uint32_t h_count;   // huge-degree vertices of this level
-  uint32_t pad[29];
+// Per-level control slot; four rotate: level L appends into slot L%4's pool
+// cursor and huge list, the consumer of level L+1 reads it, and the slot is
+// zeroed during level L+2 for reuse at level L+4.
+struct alignas(256) LevelSlot {
+  uint32_t pool_cursor;  // pool blocks taken by level L's output
+  uint32_t h_count;      // huge-degree vertices in level L's frontier
+  uint32_t pad[62];
 };
 
 struct Control {
-  LevelSlot slot[3];
+  GridBarrier bar;
+  LevelSlot slot[4];
   uint32_t depth;     // levels executed (max level + 1)
-  uint32_t pad0;
-  uint64_t reached;   // vertices placed in any frontier
-  uint32_t pad1[28];
+  uint32_t err;       // nonzero: a queue capacity was exceeded
+  uint64_t reached;   // vertices with a level
+  uint32_t bu_levels; // levels run bottom-up
+  uint32_t pad[59];
+};
+
+// Profiling counters, one row per level (optional).
+enum ProfField : int {
+  kProfFrontier = 0,  // vertices in the level's frontier
+  kProfEdges,         // CSR entries read by the expansion
+  kProfProbes,        // visited / frontier-bitmap tests
+  kProfAtomics,       // atomic OR attempts on the visited bitmap
+  kProfClaims,        // vertices given the next level
+  kProfHuge,          // frontier vertices expanded by the whole grid
+  kProfBottomUp,      // 1 when the level ran bottom-up
+  kProfFields = 8
 };
 
 struct BfsArgs {
@@ -62,15 +76,23 @@ struct BfsArgs {
   uint32_t *level;       // [n]
   uint32_t *parent;      // [n] or null
   uint32_t *vis;         // visited bitmap [n_words]
-  uint32_t *q[2];        // frontier queues [n] each
+  uint32_t *fb;          // frontier bitmap [n_words + 1] (bottom-up levels)
+  QueueBuf q[2];         // frontier queue buffers
   uint32_t *hq;          // huge-degree queue [n]
   Control *ctl;
+  uint64_t *prof;        // [prof_levels][kProfFields] or null
+  uint32_t prof_levels;
   uint32_t n;
   uint32_t n_words;
   uint32_t source;
   uint32_t small_max;    // <= 64
   uint32_t medium_max;
   uint32_t large_max;
+  uint32_t home_blocks;  // = gridDim.x * waves per workgroup
+  uint32_t pool_cap;     // pool blocks per buffer
+  uint32_t bottom_up;    // direction optimisation allowed (symmetric graph)
+  uint32_t bu_enter;     // switch to bottom-up when the frontier has more vertices
+  uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer
 };
 
 template <int BLOCK>
@@ -78,8 +100,8 @@ struct KernelShape {
   static constexpr int kWaves = BLOCK / kWave;
   static constexpr int kMedCap = 2 * BLOCK;  // medium LDS queue
   static constexpr int kBigCap = BLOCK;      // large LDS queue
-  static constexpr int kStage = 256;         // per-wave next-frontier staging
   static constexpr int kHTile = 1024;        // huge vertices per phase-H tile
+  static constexpr int kMyBlocks = 64;       // input blocks per pass
 };
 
 template <int BLOCK>
@@ -109,15 +131,13 @@ struct SharedAll {
     SharedA<BLOCK> a;
     SharedH<BLOCK> h;
   } u;
-  uint32_t stage[KernelShape<BLOCK>::kWaves][KernelShape<BLOCK>::kStage];
-  uint32_t stage_n[KernelShape<BLOCK>::kWaves];
-  uint32_t stage_base[KernelShape<BLOCK>::kWaves];
-  uint32_t med_n, big_n, chunk_base;
+  uint32_t my_blk[KernelShape<BLOCK>::kMyBlocks];
+  uint32_t my_pre[KernelShape<BLOCK>::kMyBlocks + 1];
+  uint32_t wave_sum[KernelShape<BLOCK>::kWaves];
+  uint64_t prof[kProfFields];
+  uint64_t bcast;
+  uint32_t med_n, big_n;
 };
-
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Wave-uniform gather of the edges of up to 64 lane-owned vertices.
 template <int BLOCK>
@@ -185,150 +205,132 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
   }
 }
 
-// Moves every wave's staged entries to the HBM queue with one atomic per
-// workgroup.  Must be reached by all threads of the workgroup.
+// Sum over the workgroup of a wave-uniform value (thread 0 gets the total in
+// the return value, others get garbage).  Ends with a __syncthreads.
 template <int BLOCK>
-__device__ __forceinline__ void flush_block(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t wave) {
-  constexpr int W = KernelShape<BLOCK>::kWaves;
-  if (lane_id() == 0) sh.stage_n[wave] = vis.out_n;
+__device__ __forceinline__ uint32_t block_sum_uniform(SharedAll<BLOCK> &sh, uint32_t v, uint32_t wave) {
+  if (lane_id() == 0) sh.wave_sum[wave] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t tot = 0;
-    for (int i = 0; i < W; ++i) {
-      sh.stage_base[i] = tot;
-      tot += sh.stage_n[i];
-    }
-    uint32_t base = 0;
-    if (tot) base = __hip_atomic_fetch_add(vis.q_count, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int i = 0; i < W; ++i) sh.stage_base[i] += base;
-  }
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < KernelShape<BLOCK>::kWaves; ++i) t += sh.wave_sum[i];
   __syncthreads();
-  const uint32_t cnt = vis.out_n, base = sh.stage_base[wave];
-  for (uint32_t k = lane_id(); k < cnt; k += kWave) vis.qout[base + k] = vis.stage[k];
-  vis.out_n = 0;
-  __syncthreads();
+  return t;
+}
+
+// Sum over all lanes of the workgroup of a per-lane value into sh.prof[f]
+// (profiling only).
+template <int BLOCK>
+__device__ __forceinline__ void prof_add(SharedAll<BLOCK> &sh, int f, uint64_t v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  if (lane_id() == 0) atomicAdd((unsigned long long *)&sh.prof[f], (unsigned long long)v);
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
+__device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &sh, uint32_t L) {
+  __syncthreads();
+  if (threadIdx.x < kProfFields) {
+    const uint64_t v = sh.prof[threadIdx.x];
+    if (v && L < a.prof_levels)
+      atomicAdd((unsigned long long *)&a.prof[(size_t)L * kProfFields + threadIdx.x],
+                (unsigned long long)v);
+    sh.prof[threadIdx.x] = 0;
+  }
+  __syncthreads();
+}
+
+// Top-down phase A over this workgroup's share of the input blocks.
+template <int BLOCK>
+__device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
+                                const QueueBuf &in, uint32_t pool_in, LevelSlot &cur,
+                                uint32_t wave, uint64_t &edges) {
   using S = KernelShape<BLOCK>;
-  __shared__ SharedAll<BLOCK> sh;
-  cg::grid_group grid = cg::this_grid();
-
   const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / kWave;
-  const uint32_t gtid = blockIdx.x * BLOCK + tid;
-  const uint32_t gstride = gridDim.x * BLOCK;
-
-  // ---- initialisation (reference :216 cost init, done on the device) ----
-  for (uint32_t i = gtid; i < a.n; i += gstride) {
-    a.level[i] = (i == a.source) ? 0u : kInf;
-    if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t H = a.home_blocks;
+  // my blocks: home blocks b + i*G (i < waves) then pool blocks b + i*G
+  const uint32_t n_home = (uint32_t)S::kWaves;
+  const uint32_t n_pool = pool_in > b ? (pool_in - b - 1) / G + 1 : 0;
+  const uint32_t n_mine = n_home + n_pool;
+  if (tid == 0) {
+    sh.med_n = 0;
+    sh.big_n = 0;
   }
-  for (uint32_t w = gtid; w < a.n_words; w += gstride)
-    a.vis[w] = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
-  if (gtid == 0) {
-    for (int s = 0; s < 3; ++s) {
-      a.ctl->slot[s].q_count = 0;
-      a.ctl->slot[s].q_cursor = 0;
-      a.ctl->slot[s].h_count = 0;
-    }
-    a.ctl->slot[0].q_count = 1;
-    a.q[0][0] = a.source;
-  }
-  grid.sync();
-
-  Visitor vis;
-  vis.col = a.col;
-  vis.level = a.level;
-  vis.parent = a.parent;
-  vis.vis = a.vis;
-  vis.stage = sh.stage[wave];
-  vis.stage_cap = S::kStage;
-  vis.out_n = 0;
-
-  uint64_t reached = 0;
-  for (uint32_t L = 0;; ++L) {
-    LevelSlot &cur = a.ctl->slot[L % 3];
-    LevelSlot &nxt = a.ctl->slot[(L + 1) % 3];
-    if (gtid == 0) {
-      LevelSlot &z = a.ctl->slot[(L + 2) % 3];
-      z.q_count = 0;
-      z.q_cursor = 0;
-      z.h_count = 0;
-    }
-    const uint32_t n_in = ld_agent(&cur.q_count);
-    reached += n_in;
-    const uint32_t *qin = a.q[L & 1];
-    vis.qout = a.q[(L + 1) & 1];
-    vis.q_count = &nxt.q_count;
-    vis.next_level = L + 1;
-
-    // ---- phase A: frontier chunks -> bins -> expansion ----
-    if (tid == 0) {
-      sh.med_n = 0;
-      sh.big_n = 0;
+  for (uint32_t p0 = 0; p0 < n_mine; p0 += S::kMyBlocks) {
+    const uint32_t np = min((uint32_t)S::kMyBlocks, n_mine - p0);
+    __syncthreads();
+    if (tid < (uint32_t)kWave) {
+      uint32_t c = 0, blk = 0;
+      if (tid < np) {
+        const uint32_t i = p0 + tid;
+        blk = i < n_home ? b + i * G : H + b + (i - n_home) * G;
+        c = in.cnt[blk];
+      }
+      const uint32_t incl = wave_inclusive_scan(c);
+      if (tid < np) {
+        sh.my_blk[tid] = blk;
+        sh.my_pre[tid] = incl - c;
+      }
+      if (tid == kWave - 1) sh.my_pre[np] = incl;
     }
     __syncthreads();
-    for (;;) {
-      if (tid == 0)
-        sh.chunk_base = __hip_atomic_fetch_add(&cur.q_cursor, (uint32_t)BLOCK, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-      __syncthreads();
-      const uint32_t base = sh.chunk_base;
-      if (base >= n_in) break;
-      const uint32_t i = base + tid;
+    const uint32_t total = sh.my_pre[np];
+    for (uint32_t base = 0; base < total; base += BLOCK) {
+      const uint32_t p = base + tid;
       uint32_t v = 0, deg = 0;
       int64_t start = 0;
-      if (i < n_in) {
-        v = qin[i];
+      if (p < total) {
+        uint32_t j = 0;
+        for (uint32_t step = S::kMyBlocks / 2; step >= 1; step >>= 1)
+          if (j + step < np && sh.my_pre[j + step] <= p) j += step;
+        v = in.ent[(size_t)sh.my_blk[j] * kBlk + (p - sh.my_pre[j])];
         start = a.off[v];
         deg = (uint32_t)(a.off[v + 1] - start);
+        edges += deg;
       }
       const bool is_small = deg != 0 && deg <= a.small_max;
       const bool is_med = deg > a.small_max && deg <= a.medium_max;
       const bool is_big = deg > a.medium_max && deg <= a.large_max;
       const bool is_huge = deg > a.large_max;
-      // wave-aggregated LDS slot reservation for the medium / large bins
-      {
-        const uint64_t mm = __ballot(is_med);
-        if (mm) {
-          uint32_t b0 = 0;
-          if (lane_id() == 0) b0 = atomicAdd(&sh.med_n, (uint32_t)__popcll(mm));
-          b0 = bcast_lane(b0, 0);
-          if (is_med) {
-            const uint32_t p = b0 + mbcnt(mm);
-            sh.u.a.med_v[p] = v;
-            sh.u.a.med_d[p] = deg;
-            sh.u.a.med_s[p] = start;
-          }
+      // wave-aggregated slot reservation (ballot + mbcnt) for the bins
+      const uint64_t mm = __ballot(is_med);
+      if (mm) {
+        uint32_t b0 = 0;
+        if (lane_id() == 0) b0 = atomicAdd(&sh.med_n, (uint32_t)__popcll(mm));
+        b0 = bcast_lane(b0, 0);
+        if (is_med) {
+          const uint32_t q = b0 + mbcnt(mm);
+          sh.u.a.med_v[q] = v;
+          sh.u.a.med_d[q] = deg;
+          sh.u.a.med_s[q] = start;
         }
-        const uint64_t bm = __ballot(is_big);
-        if (bm) {
-          uint32_t b0 = 0;
-          if (lane_id() == 0) b0 = atomicAdd(&sh.big_n, (uint32_t)__popcll(bm));
-          b0 = bcast_lane(b0, 0);
-          if (is_big) {
-            const uint32_t p = b0 + mbcnt(bm);
-            sh.u.a.big_v[p] = v;
-            sh.u.a.big_d[p] = deg;
-            sh.u.a.big_s[p] = start;
-          }
+      }
+      const uint64_t bm = __ballot(is_big);
+      if (bm) {
+        uint32_t b0 = 0;
+        if (lane_id() == 0) b0 = atomicAdd(&sh.big_n, (uint32_t)__popcll(bm));
+        b0 = bcast_lane(b0, 0);
+        if (is_big) {
+          const uint32_t q = b0 + mbcnt(bm);
+          sh.u.a.big_v[q] = v;
+          sh.u.a.big_d[q] = deg;
+          sh.u.a.big_s[q] = start;
         }
-        const uint64_t hm = __ballot(is_huge);
-        if (hm) {
-          uint32_t b0 = 0;
-          if (lane_id() == 0)
-            b0 = __hip_atomic_fetch_add(&cur.h_count, (uint32_t)__popcll(hm), __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-          b0 = bcast_lane(b0, 0);
-          if (is_huge) a.hq[b0 + mbcnt(hm)] = v;
-        }
+      }
+      const uint64_t hm = __ballot(is_huge);
+      if (hm) {
+        uint32_t b0 = 0;
+        if (lane_id() == 0)
+          b0 = __hip_atomic_fetch_add(&cur.h_count, (uint32_t)__popcll(hm), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        b0 = bcast_lane(b0, 0);
+        if (is_huge) a.hq[b0 + mbcnt(hm)] = v;
       }
       gather_small<BLOCK>(vis, is_small ? v : 0u, is_small ? start : 0, is_small ? deg : 0u,
                           sh.u.a.g_excl[wave], sh.u.a.g_v[wave], sh.u.a.g_s[wave]);
       __syncthreads();
       const uint32_t mn = sh.med_n, bn = sh.big_n;
+      __syncthreads();
       if (mn > (uint32_t)(S::kMedCap - BLOCK)) {
         expand_medium<BLOCK>(vis, sh, mn, wave);
         __syncthreads();
@@ -341,83 +343,247 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
       }
       __syncthreads();
     }
-    // drain (reference :161-169)
-    expand_medium<BLOCK>(vis, sh, sh.med_n, wave);
-    expand_large<BLOCK>(vis, sh, sh.big_n);
-    flush_block<BLOCK>(vis, sh, wave);
-    grid.sync();
+  }
+  __syncthreads();
+  // drain (reference :161-169)
+  const uint32_t mn = sh.med_n, bn = sh.big_n;
+  expand_medium<BLOCK>(vis, sh, mn, wave);
+  expand_large<BLOCK>(vis, sh, bn);
+}
 
-    // ---- phase H: huge-degree vertices over the whole grid ----
-    const uint32_t hc = ld_agent(&cur.h_count);
-    if (hc) {
-      constexpr int HT = S::kHTile;
-      for (uint32_t t0 = 0; t0 < hc; t0 += HT) {
-        const uint32_t tn = min((uint32_t)HT, hc - t0);
-        // degrees of the tile and their exclusive prefix (per-thread runs of
-        // HT/BLOCK entries, wave scan of run totals, cross-wave add)
-        constexpr int RUN = HT / BLOCK;
-        uint64_t run = 0;
-        for (int r = 0; r < RUN; ++r) {
-          const uint32_t idx = tid * RUN + r;
-          uint32_t hvtx = 0, d = 0;
-          int64_t s = 0;
-          if (idx < tn) {
-            hvtx = a.hq[t0 + idx];
-            s = a.off[hvtx];
-            d = (uint32_t)(a.off[hvtx + 1] - s);
-          }
-          sh.u.h.hv[idx] = hvtx;
-          sh.u.h.hs[idx] = s;
-          sh.u.h.hx[idx] = run;  // exclusive within the run for now
-          run += d;
-        }
-        // 64-bit inclusive scan of the run totals across the wave, then
-        // across waves through LDS.
-        uint64_t incl = run;
-        for (int d = 1; d < kWave; d <<= 1) {
-          const uint64_t t = __shfl_up(incl, d);
-          if (lane_id() >= (uint32_t)d) incl += t;
-        }
-        if (lane_id() == kWave - 1) sh.u.h.wave_tot[wave] = incl;
-        __syncthreads();
-        uint64_t wave_off = 0, tot = 0;
-        for (int w = 0; w < S::kWaves; ++w) {
-          if ((uint32_t)w < wave) wave_off += sh.u.h.wave_tot[w];
-          tot += sh.u.h.wave_tot[w];
-        }
-        const uint64_t excl = wave_off + (incl - run);
-        for (int r = 0; r < RUN; ++r) sh.u.h.hx[tid * RUN + r] += excl;
-        __syncthreads();
-        // Entries beyond tn have degree 0 and an exclusive prefix == tot, so
-        // the owner search below never selects them.
-        for (uint64_t ch = blockIdx.x; ch * BLOCK < tot; ch += gridDim.x) {
-          const uint64_t e = ch * BLOCK + tid;
-          const bool act = e < tot;
-          uint32_t j = 0;
-#pragma unroll
-          for (uint32_t step = HT / 2; step >= 1; step >>= 1)
-            if (sh.u.h.hx[j + step] <= e) j += step;
-          uint32_t w = 0, u = 0;
-          if (act) {
-            u = sh.u.h.hv[j];
-            w = a.col[sh.u.h.hs[j] + (int64_t)(e - sh.u.h.hx[j])];
-          }
-          vis.visit(act, u, w);
-        }
-        __syncthreads();
+// Phase H: huge-degree vertices, edges split evenly over the whole grid.
+template <int BLOCK>
+__device__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t hc,
+                                uint32_t wave, uint64_t &edges) {
+  using S = KernelShape<BLOCK>;
+  constexpr int HT = S::kHTile;
+  constexpr int RUN = HT / BLOCK;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t t0 = 0; t0 < hc; t0 += HT) {
+    const uint32_t tn = min((uint32_t)HT, hc - t0);
+    __syncthreads();
+    uint64_t run = 0;
+    for (int r = 0; r < RUN; ++r) {
+      const uint32_t idx = tid * RUN + r;
+      uint32_t hv = 0, d = 0;
+      int64_t s = 0;
+      if (idx < tn) {
+        hv = a.hq[t0 + idx];
+        s = a.off[hv];
+        d = (uint32_t)(a.off[hv + 1] - s);
       }
-      flush_block<BLOCK>(vis, sh, wave);
-      grid.sync();
+      sh.u.h.hv[idx] = hv;
+      sh.u.h.hs[idx] = s;
+      sh.u.h.hx[idx] = run;
+      run += d;
     }
+    uint64_t incl = run;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint64_t t = __shfl_up(incl, d);
+      if (lane_id() >= (uint32_t)d) incl += t;
+    }
+    if (lane_id() == kWave - 1) sh.u.h.wave_tot[wave] = incl;
+    __syncthreads();
+    uint64_t wave_off = 0, tot = 0;
+    for (int w = 0; w < S::kWaves; ++w) {
+      if ((uint32_t)w < wave) wave_off += sh.u.h.wave_tot[w];
+      tot += sh.u.h.wave_tot[w];
+    }
+    const uint64_t excl = wave_off + (incl - run);
+    for (int r = 0; r < RUN; ++r) sh.u.h.hx[tid * RUN + r] += excl;
+    __syncthreads();
+    if (blockIdx.x == 0 && tid == 0) edges += tot;
+    // Entries beyond tn have degree 0 and prefix == tot: never selected.
+    for (uint64_t ch = blockIdx.x; ch * BLOCK < tot; ch += gridDim.x) {
+      const uint64_t e = ch * BLOCK + tid;
+      const bool act = e < tot;
+      uint32_t j = 0;
+#pragma unroll
+      for (uint32_t step = HT / 2; step >= 1; step >>= 1)
+        if (sh.u.h.hx[j + step] <= e) j += step;
+      uint32_t w = 0, u = 0;
+      if (act) {
+        u = sh.u.h.hv[j];
+        w = a.col[sh.u.h.hs[j] + (int64_t)(e - sh.u.h.hx[j])];
+      }
+      vis.visit(act, u, w);
+    }
+  }
+  __syncthreads();
+}
 
-    const uint32_t n_next = ld_agent(&nxt.q_count);
+// Bottom-up pass 1: frontier bitmap fb = { v : level[v] == L } (one 64-vertex
+// group per wave step, coalesced level reads).
+__device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t L, uint32_t gwave,
+                                                      uint32_t nwaves) {
+  const uint32_t groups = (a.n + 63) / 64;
+  for (uint32_t g = gwave; g < groups; g += nwaves) {
+    const uint32_t v = g * 64 + lane_id();
+    const bool f = v < a.n && a.level[v] == L;
+    const uint64_t m = __ballot(f);
+    if (lane_id() == 0) a.fb[2 * g] = (uint32_t)m;
+    if (lane_id() == 1 && 2 * g + 1 < a.n_words + 1) a.fb[2 * g + 1] = (uint32_t)(m >> 32);
+  }
+}
+
+// Bottom-up pass 2: every unvisited vertex scans its neighbours for one in
+// the frontier; the owning wave updates the visited words directly.
+__device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, uint32_t L,
+                                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
+  const uint32_t groups = (a.n + 63) / 64;
+  for (uint32_t g = gwave; g < groups; g += nwaves) {
+    const uint32_t v = g * 64 + lane_id();
+    const uint32_t w0 = a.vis[2 * g];
+    const uint32_t w1 = 2 * g + 1 < a.n_words ? a.vis[2 * g + 1] : 0xffffffffu;
+    const uint64_t seen = ((uint64_t)w1 << 32) | w0;
+    bool claimed = false;
+    uint32_t par = 0;
+    if (v < a.n && !((seen >> lane_id()) & 1ull)) {
+      const int64_t s = a.off[v], e = a.off[v + 1];
+      for (int64_t k = s; k < e; ++k) {
+        const uint32_t w = a.col[k];
+        ++vis.probes;
+        ++edges;
+        if ((a.fb[w >> 5] >> (w & 31u)) & 1u) {
+          claimed = true;
+          par = w;
+          break;
+        }
+      }
+    }
+    if (claimed) {
+      a.level[v] = L + 1;
+      if (a.parent) a.parent[v] = par;
+    }
+    const uint64_t mask = __ballot(claimed);
+    if (mask) {
+      if (lane_id() == 0) a.vis[2 * g] = w0 | (uint32_t)mask;
+      if (lane_id() == 1 && 2 * g + 1 < a.n_words) a.vis[2 * g + 1] = w1 | (uint32_t)(mask >> 32);
+      vis.append(claimed, mask, v);
+    }
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
+  using S = KernelShape<BLOCK>;
+  __shared__ SharedAll<BLOCK> sh;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / kWave;
+  const uint32_t gtid = blockIdx.x * BLOCK + tid;
+  const uint32_t gstride = gridDim.x * BLOCK;
+  const uint32_t gwave = blockIdx.x * S::kWaves + wave;
+  const uint32_t nwaves = gridDim.x * S::kWaves;
+  Control *ctl = a.ctl;
+
+  BarrierState bst{0, 0, 0, 0};
+  if (tid == 0) barrier_census(&ctl->bar, bst);
+  if (tid < kProfFields) sh.prof[tid] = 0;
+
+  // ---- initialisation (reference :216 cost init, done on the device) ----
+  for (uint32_t i = gtid; i < a.n; i += gstride) {
+    a.level[i] = (i == a.source) ? 0u : kInf;
+    if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
+  }
+  for (uint32_t w = gtid; w < a.n_words; w += gstride)
+    a.vis[w] = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
+  // level-0 frontier: home block of wave 0 holds the source
+  if (lane_id() == 0) a.q[0].cnt[gwave] = (gwave == 0) ? 1u : 0u;
+  if (gtid == 0) {
+    a.q[0].ent[0] = a.source;
+    for (int s = 0; s < 4; ++s) {
+      ctl->slot[s].pool_cursor = 0;
+      ctl->slot[s].h_count = 0;
+    }
+    ctl->err = 0;
+  }
+  grid_barrier(&ctl->bar, bst, 0, &sh.bcast);
+
+  Visitor vis;
+  vis.col = a.col;
+  vis.level = a.level;
+  vis.parent = a.parent;
+  vis.vis = a.vis;
+  vis.home_blocks = a.home_blocks;
+  vis.pool_cap = a.pool_cap;
+  vis.err = &ctl->err;
+  vis.probes = 0;
+  vis.atomics = 0;
+
+  uint64_t n_cur = 1, n_prev = 0, reached = 1;
+  uint32_t bu_levels = 0;
+  bool bottom_up = false;
+  for (uint32_t L = 0;; ++L) {
+    const QueueBuf in = a.q[L & 1];
+    LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
+    LevelSlot &cur = ctl->slot[L & 3];
+    if (gtid == 0) {
+      ctl->slot[(L + 2) & 3].pool_cursor = 0;
+      ctl->slot[(L + 2) & 3].h_count = 0;
+    }
+    // direction choice: identical inputs in every workgroup
+    if (a.bottom_up) {
+      if (!bottom_up && n_cur > a.bu_enter && n_cur > n_prev) bottom_up = true;
+      else if (bottom_up && n_cur < a.bu_exit && n_cur < n_prev) bottom_up = false;
+    }
+    vis.out = a.q[(L + 1) & 1];
+    vis.pool_cursor = &cur.pool_cursor;
+    vis.begin_level(gwave, L + 1);
+    uint64_t edges = 0;
+    uint64_t n_next;
+    uint32_t hc = 0;
+    if (!bottom_up) {
+      // the cursor can pass pool_cap only when err is already set
+      const uint32_t pool_in = min(ld_agent(&in_slot.pool_cursor), a.pool_cap);
+      topdown_phase_a<BLOCK>(a, sh, vis, in, pool_in, cur, wave, edges);
+      vis.publish();
+      const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
+      n_next = grid_barrier(&ctl->bar, bst, wg_claims, &sh.bcast);
+      hc = ld_agent(&cur.h_count);
+      if (hc) {
+        const uint32_t before = vis.claims;
+        topdown_phase_h<BLOCK>(a, sh, vis, hc, wave, edges);
+        vis.publish();
+        const uint32_t wg2 = block_sum_uniform<BLOCK>(sh, vis.claims - before, wave);
+        n_next += grid_barrier(&ctl->bar, bst, wg2, &sh.bcast);
+      }
+    } else {
+      ++bu_levels;
+      build_frontier_bitmap(a, L, gwave, nwaves);
+      grid_barrier(&ctl->bar, bst, 0, &sh.bcast);
+      bottomup_expand(a, vis, L, gwave, nwaves, edges);
+      vis.publish();
+      const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
+      n_next = grid_barrier(&ctl->bar, bst, wg_claims, &sh.bcast);
+    }
+    if (a.prof) {
+      prof_add<BLOCK>(sh, kProfEdges, edges);
+      prof_add<BLOCK>(sh, kProfProbes, vis.probes);
+      prof_add<BLOCK>(sh, kProfAtomics, vis.atomics);
+      if (tid == 0 && blockIdx.x == 0) {
+        sh.prof[kProfFrontier] = n_cur;
+        sh.prof[kProfClaims] = n_next;
+        sh.prof[kProfHuge] = hc;
+        sh.prof[kProfBottomUp] = bottom_up ? 1 : 0;
+      }
+      prof_flush<BLOCK>(a, sh, L);
+      vis.probes = 0;
+      vis.atomics = 0;
+    }
+    reached += n_next;
     if (n_next == 0) {
       if (gtid == 0) {
-        a.ctl->depth = L + 1;
-        a.ctl->reached = reached;
+        ctl->depth = L + 1;
+        ctl->reached = reached;
+        ctl->bu_levels = bu_levels;
+        // every workgroup read the census before the first barrier
+        barrier_census_clear(&ctl->bar);
       }
       break;
     }
+    n_prev = n_cur;
+    n_cur = n_next;
   }
 }
 

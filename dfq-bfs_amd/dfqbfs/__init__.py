@@ -67,7 +67,9 @@ class _Csr(C.Structure):
 class Config(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("small_max", C.c_uint32), ("medium_max", C.c_uint32),
-                ("large_max", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("large_max", C.c_uint32), ("direction_opt", C.c_uint32),
+                ("bu_enter_div", C.c_uint32), ("bu_exit_div", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -75,7 +77,8 @@ class Config(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("depth", C.c_uint32), ("grid_blocks", C.c_uint32),
-                ("block_size", C.c_uint32), ("reserved0", C.c_uint32), ("reached", C.c_uint64)]
+                ("block_size", C.c_uint32), ("bottom_up_levels", C.c_uint32),
+                ("reached", C.c_uint64)]
 
 
 class DeviceInfo(C.Structure):
@@ -135,6 +138,15 @@ _sig("dfq_graph_get_config", C.c_int, _P, C.POINTER(Config))
 _sig("dfq_graph_device_arrays", C.c_int, _P, C.POINTER(_P), C.POINTER(_P))
 _sig("dfq_bfs", C.c_int, _P, C.c_uint32, _u32p, _u32p, C.POINTER(Stats))
 _sig("dfq_bfs_device", C.c_int, _P, C.c_uint32, _P, _P, C.POINTER(Stats))
+_sig("dfq_bfs_profile", C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32,
+     C.POINTER(Stats))
+_sig("dfq_graph_is_symmetric", C.c_int, _P)
+_sig("dfq_device_alloc", C.c_int, C.c_size_t, C.POINTER(_P))
+_sig("dfq_device_free", C.c_int, _P)
+_sig("dfq_copy_to_host", C.c_int, _P, _P, C.c_size_t)
+_sig("dfq_copy_to_device", C.c_int, _P, _P, C.c_size_t)
+_sig("dfq_device_synchronize", C.c_int)
+_sig("dfq_stream_copy", C.c_int, _P, _P, C.c_size_t, C.POINTER(C.c_double))
 
 
 def check(rc: int, what: str):
@@ -293,6 +305,7 @@ class BfsResult:
     reached: int
     grid_blocks: int
     block_size: int
+    bottom_up_levels: int = 0
 
 
 class DeviceGraph:
@@ -346,7 +359,22 @@ class DeviceGraph:
         check(lib.dfq_bfs(self._h, source, _u32(level), _u32(parent) if parents else None,
                           C.byref(st)), "dfq_bfs")
         return BfsResult(level, parent, st.kernel_ms, st.depth, st.reached, st.grid_blocks,
-                         st.block_size)
+                         st.block_size, st.bottom_up_levels)
+
+    @property
+    def symmetric(self) -> bool:
+        return bool(lib.dfq_graph_is_symmetric(self._h))
+
+    PROFILE_FIELDS = ("frontier", "edges", "probes", "atomics", "claims", "huge", "bottom_up")
+
+    def profile(self, source: int, max_levels: int = 64):
+        """Per-level counters: list of dicts (levels beyond max_levels dropped)."""
+        rows = np.zeros((max_levels, 8), np.uint64)
+        st = Stats()
+        check(lib.dfq_bfs_profile(self._h, source, rows.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                  max_levels, C.byref(st)), "dfq_bfs_profile")
+        depth = min(st.depth, max_levels)
+        return [dict(zip(self.PROFILE_FIELDS, map(int, rows[i, :7]))) for i in range(depth)], st
 
     def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
         """BFS into caller-owned device buffers (raw device addresses)."""
@@ -355,3 +383,48 @@ class DeviceGraph:
                                  C.c_void_p(d_parent) if d_parent else None, C.byref(st)),
               "dfq_bfs_device")
         return st
+
+
+class DeviceBuffer:
+    """Device allocation owned through the library's HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        p = _P()
+        check(lib.dfq_device_alloc(nbytes, C.byref(p)), "dfq_device_alloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def to_numpy(self, dtype=np.uint32, count=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        count = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(count, dt)
+        check(lib.dfq_copy_to_host(out.ctypes.data_as(_P), C.c_void_p(self.ptr), out.nbytes),
+              "dfq_copy_to_host")
+        return out
+
+    def from_numpy(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than buffer")
+        check(lib.dfq_copy_to_device(C.c_void_p(self.ptr), a.ctypes.data_as(_P), a.nbytes),
+              "dfq_copy_to_device")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib.dfq_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def synchronize():
+    check(lib.dfq_device_synchronize(), "dfq_device_synchronize")
+
+
+def stream_copy(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> float:
+    """Copy nbytes device-to-device with the library's streaming kernel; ms."""
+    ms = C.c_double()
+    check(lib.dfq_stream_copy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, C.byref(ms)),
+          "dfq_stream_copy")
+    return ms.value

@@ -38,26 +38,36 @@ extern "C" {
 
 /* Tuning knobs of the persistent kernel (the reference fixed these with
  * #defines, bfs.cu:16-36 upper-bound, lower-bound/bfs.cu:16-41).
- * Degree bins: deg <= small_max       -> lane-owned, wave-gathered edges
+ * Degree bins of a top-down level:
+ *              deg <= small_max       -> lane-owned, wave-gathered edges
  *              deg <= medium_max      -> one wavefront (64 lanes) per vertex
  *              deg <= large_max       -> one workgroup per vertex
- *              deg >  large_max       -> whole grid, HBM queue                */
+ *              deg >  large_max       -> whole grid, HBM queue
+ * direction_opt = 1 lets a level run bottom-up (every unvisited vertex looks
+ * for a parent in the frontier) when the frontier holds more than
+ * n / bu_enter_div vertices and is growing, and returns to top-down when a
+ * shrinking frontier holds fewer than n / bu_exit_div.  It only takes effect
+ * on symmetric graphs (checked when the graph is created); levels are the
+ * same either way.  direction_opt = 0 runs every level top-down. */
 typedef struct dfq_config {
-  uint32_t block_size;     /* threads per workgroup: 256 or 512           */
+  uint32_t block_size;     /* threads per workgroup: 256, 512 or 1024     */
   uint32_t blocks_per_cu;  /* 0 = as many as occupancy allows             */
   uint32_t small_max;      /* <= 64                                        */
   uint32_t medium_max;     /* >= small_max                                 */
   uint32_t large_max;      /* >= medium_max                                */
-  uint32_t reserved[3];    /* must be zero                                 */
+  uint32_t direction_opt;  /* 0 or 1                                       */
+  uint32_t bu_enter_div;   /* > 0 when direction_opt                       */
+  uint32_t bu_exit_div;    /* > 0 when direction_opt                       */
+  uint32_t reserved;       /* must be zero                                 */
 } dfq_config;
 
 typedef struct dfq_stats {
-  double   kernel_ms;      /* hipEvent time of the cooperative launch     */
-  uint32_t depth;          /* number of BFS levels (max level + 1)        */
-  uint32_t grid_blocks;    /* workgroups launched                         */
-  uint32_t block_size;     /* threads per workgroup                       */
-  uint32_t reserved0;
-  uint64_t reached;        /* vertices with a level                       */
+  double   kernel_ms;        /* hipEvent time of the cooperative launch   */
+  uint32_t depth;            /* number of BFS levels (max level + 1)      */
+  uint32_t grid_blocks;      /* workgroups launched                       */
+  uint32_t block_size;       /* threads per workgroup                     */
+  uint32_t bottom_up_levels; /* levels that ran bottom-up                 */
+  uint64_t reached;          /* vertices with a level                     */
 } dfq_stats;
 
 typedef struct dfq_device_info {
@@ -101,11 +111,43 @@ int dfq_graph_device_arrays(const dfq_graph *g, const int64_t **d_row_offsets,
 int dfq_bfs(dfq_graph *g, uint32_t source, uint32_t *level_out,
             uint32_t *parent_out, dfq_stats *stats);
 
+/* 1 when the graph is symmetric (every u->v has v->u, rows sorted), which
+ * is what bottom-up levels need; 0 otherwise. */
+int dfq_graph_is_symmetric(const dfq_graph *g);
+
 /* Same BFS with outputs left in device memory (d_level[n] required,
  * d_parent[n] optional).  Blocks until the kernel has finished; the timed
  * region of stats->kernel_ms is the cooperative launch only. */
 int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
                    uint32_t *d_parent, dfq_stats *stats);
+
+/* BFS with per-level counters (the reference's frontier profiling build,
+ * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,140-144,
+ * 200-209).  rows[max_levels][8], one row per level:
+ *   0 frontier vertices, 1 CSR entries read, 2 bitmap tests,
+ *   3 atomic OR attempts, 4 vertices claimed, 5 grid-expanded vertices,
+ *   6 1 if the level ran bottom-up, 7 unused.
+ * Levels beyond max_levels are not recorded.  The counters slow the search;
+ * time it with dfq_bfs_device instead. */
+int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows,
+                    uint32_t max_levels, dfq_stats *stats);
+
+/* ---- device memory helpers ----------------------------------------------
+ * No counterpart in the reference (it used cudaMallocManaged, bfs.cu:186,
+ * 194-197).  They let callers that hold no HIP runtime of their own (the
+ * ctypes tests and benchmark) own device buffers for dfq_bfs_device. */
+int dfq_device_alloc(size_t bytes, void **d_ptr);
+int dfq_device_free(void *d_ptr);
+int dfq_copy_to_host(void *h_dst, const void *d_src, size_t bytes);
+int dfq_copy_to_device(void *d_dst, const void *h_src, size_t bytes);
+int dfq_device_synchronize(void);
+
+/* Plain device-to-device copy of `bytes` (multiple of 16) through a
+ * streaming HIP kernel, timed with hipEvents.  No counterpart in the
+ * reference: it is the known-size workload against which HBM traffic
+ * counters (FETCH_SIZE / WRITE_SIZE) of the BFS kernel are calibrated, and
+ * it gives the attainable copy bandwidth on the running device. */
+int dfq_stream_copy(void *d_dst, const void *d_src, size_t bytes, double *ms_out);
 
 #ifdef __cplusplus
 }

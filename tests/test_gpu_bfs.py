@@ -42,6 +42,8 @@ CONFIGS = [
     dict(block_size=256, small_max=0, medium_max=4096, large_max=4096),
     dict(block_size=512, small_max=8, medium_max=64, large_max=1024),
     dict(block_size=256, small_max=16, medium_max=256, large_max=16384, blocks_per_cu=1),
+    dict(block_size=1024, small_max=32, medium_max=2048, large_max=32768),
+    dict(block_size=1024, small_max=4, medium_max=128, large_max=512),
 ]
 
 
@@ -114,6 +116,43 @@ def test_rmat_many_sources(dfq, oracle, scale):
                          oracle.bfs(csr.row_offsets, csr.col_idx, int(s)))
 
 
+DO_CONFIGS = [
+    dict(block_size=256, direction_opt=1, bu_enter_div=64, bu_exit_div=512),
+    dict(block_size=1024, direction_opt=1, bu_enter_div=256, bu_exit_div=1024),
+    dict(block_size=512, direction_opt=1, bu_enter_div=1000000, bu_exit_div=1),  # bottom-up from level 1
+]
+
+
+@pytest.mark.parametrize("ci", range(len(DO_CONFIGS)))
+def test_direction_optimizing_levels_match(dfq, oracle, ci):
+    """Bottom-up levels on symmetric graphs give the same levels."""
+    for scale, ef in ((12, 8), (16, 16)):
+        csr = dfq.Csr.rmat(scale, ef, seed=100 + scale)
+        with dfq.DeviceGraph(csr) as g:
+            assert g.symmetric
+            g.set_config(**DO_CONFIGS[ci])
+            for s in csr.pick_sources(4, seed=ci):
+                say("DO", DO_CONFIGS[ci], "rmat", scale, "source", int(s))
+                r = g.bfs(int(s), parents=True)
+                check_result(dfq, csr, int(s), r, oracle.bfs(csr.row_offsets, csr.col_idx, int(s)))
+                assert r.bottom_up_levels > 0
+    grid = dfq.Csr.grid2d(300, 200, 0.05, seed=5)
+    with dfq.DeviceGraph(grid) as g:
+        g.set_config(**DO_CONFIGS[ci])
+        say("DO grid")
+        check_result(dfq, grid, 0, g.bfs(0, parents=True), oracle.bfs(grid.row_offsets, grid.col_idx, 0))
+
+
+def test_directed_graph_never_runs_bottom_up(dfq, oracle):
+    csr = dfq.Csr.amazon_like(50000, seed=3)
+    with dfq.DeviceGraph(csr) as g:
+        assert not g.symmetric
+        g.set_config(direction_opt=1, bu_enter_div=1000000, bu_exit_div=1)
+        r = g.bfs(0, parents=True)
+        assert r.bottom_up_levels == 0
+        check_result(dfq, csr, 0, r, oracle.bfs(csr.row_offsets, csr.col_idx, 0))
+
+
 def test_amazon_like_config1(dfq, oracle):
     """Config 1 graph (262,144 vertices, directed), source 0."""
     csr = dfq.Csr.amazon_like(262144, seed=0xD0F1)
@@ -165,16 +204,17 @@ def test_repeat_runs_share_no_state(dfq, oracle):
 
 
 def test_device_outputs(dfq, oracle):
-    torch = pytest.importorskip("torch")
+    """dfq_bfs_device writes into caller-owned device buffers."""
     csr = dfq.Csr.rmat(15, 16, seed=9)
     s = int(csr.pick_sources(1, 3)[0])
-    lv = torch.empty(csr.n, dtype=torch.int32, device="cuda")
-    par = torch.empty(csr.n, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    lv = dfq.DeviceBuffer(4 * csr.n)
+    par = dfq.DeviceBuffer(4 * csr.n)
     with dfq.DeviceGraph(csr) as g:
-        st = g.bfs_device(s, lv.data_ptr(), par.data_ptr())
+        say("device outputs source", s)
+        st = g.bfs_device(s, lv.ptr, par.ptr)
         assert st.kernel_ms > 0
-    level = lv.cpu().numpy().view(np.uint32)
-    parent = par.cpu().numpy().view(np.uint32)
-    np.testing.assert_array_equal(level, oracle.bfs(csr.row_offsets, csr.col_idx, s))
-    assert csr.validate(s, level, parent)["errors"] == 0
+        level, parent = lv.to_numpy(), par.to_numpy()
+        np.testing.assert_array_equal(level, oracle.bfs(csr.row_offsets, csr.col_idx, s))
+        assert csr.validate(s, level, parent)["errors"] == 0
+        st = g.bfs_device(s, lv.ptr, None)  # parents optional
+        np.testing.assert_array_equal(lv.to_numpy(), level)

@@ -47,7 +47,7 @@ MODES = {
                     direction_opt=0, bu_enter_div=14, bu_exit_div=24),
     # the same kernel with bottom-up levels for large frontiers
     "hybrid": dict(block_size=1024, small_max=32, medium_max=2048, large_max=32768,
-                   direction_opt=1, bu_enter_div=256, bu_exit_div=1024),
+                   direction_opt=1, bu_enter_div=14, bu_exit_div=24),
 }
 
 

@@ -95,7 +95,10 @@ struct alignas(256) BarrierWord {
 struct GridBarrier {
   BarrierWord xleaf[kMaxXcd];   // per-XCD arrivals
   BarrierWord xrel[kMaxXcd];    // per-XCD release generation (low 32 bits)
+  BarrierWord xside[kMaxXcd];   // per-XCD sum of the 64-bit side payload
   BarrierWord root;             // XCD-leader arrivals
+  BarrierWord root_side;        // side payload sum over XCDs
+  BarrierWord side_out;         // side payload total of the last round
   BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
   BarrierWord census_total;     // workgroups counted at kernel start
   uint32_t census[kMaxXcd];     // workgroups per XCD
@@ -106,6 +109,7 @@ struct BarrierState {
   uint32_t xcc;     // this workgroup's XCD
   uint32_t xcc_n;   // workgroups on this XCD
   uint32_t nx;      // XCDs with workgroups
+  uint32_t rank;    // index of this XCD among those with workgroups
 };
 
 __device__ __forceinline__ uint32_t xcc_id() {
@@ -141,9 +145,14 @@ __device__ __forceinline__ void barrier_census(GridBarrier *b, BarrierState &st)
                          __HIP_MEMORY_SCOPE_AGENT);
   while (ld_relaxed(&b->census_total.word) < gridDim.x) __builtin_amdgcn_s_sleep(1);
   st.xcc_n = ld_relaxed(&b->census[st.xcc]);
-  uint32_t nx = 0;
-  for (int i = 0; i < kMaxXcd; ++i) nx += ld_relaxed(&b->census[i]) != 0;
+  uint32_t nx = 0, rank = 0;
+  for (int i = 0; i < kMaxXcd; ++i) {
+    const bool present = ld_relaxed(&b->census[i]) != 0;
+    nx += present;
+    rank += present && (uint32_t)i < st.xcc;
+  }
   st.nx = nx;
+  st.rank = rank;
 }
 
 __device__ __forceinline__ void barrier_census_clear(GridBarrier *b) {
@@ -152,28 +161,41 @@ __device__ __forceinline__ void barrier_census_clear(GridBarrier *b) {
   __hip_atomic_store(&b->census_total.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// All threads of every workgroup must call it.  `payload` is read from
-// thread 0; returns the grid-wide payload sum in every thread.  `st` is only
-// meaningful in thread 0.  `s_word` is a 64-bit LDS broadcast slot.
+// All threads of every workgroup must call it.  `payload` (< 2^40) and
+// `side` are read from thread 0; returns the grid-wide payload sum in every
+// thread and, when side_total is given, the side sum there.  `st` is only
+// meaningful in thread 0.  `s_word` points at two 64-bit LDS slots.
 __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &st, uint64_t payload,
-                                                 uint64_t *s_word) {
+                                                 uint64_t *s_word, uint64_t side = 0,
+                                                 uint64_t *side_total = nullptr) {
   wait_own_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t g = st.gen, next = (g + 1) & 0xffffffu;
+    if (side) {
+      // lands before this workgroup's arrival is counted
+      __hip_atomic_fetch_add(&b->xside[st.xcc].word, side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      wait_own_stores();
+    }
     const uint64_t old = __hip_atomic_fetch_add(&b->xleaf[st.xcc].word, kArrive | payload,
                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t total;
     if ((old >> 40) + 1 == st.xcc_n) {
       // XCD leader: every workgroup of this XCD has its stores in this L2
       const uint64_t xsum = (old & kPayloadMask) + payload;
+      const uint64_t xs = ld_relaxed(&b->xside[st.xcc].word);
       __hip_atomic_store(&b->xleaf[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      l2_writeback();
+      __hip_atomic_store(&b->xside[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (xs) __hip_atomic_fetch_add(&b->root_side.word, xs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      l2_writeback();  // also waits for the stores and atomics above
       const uint64_t o2 = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
       if ((o2 >> 40) + 1 == st.nx) {
         total = (o2 & kPayloadMask) + xsum;
+        const uint64_t rs = ld_relaxed(&b->root_side.word);
         __hip_atomic_store(&b->root.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&b->root_side.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&b->side_out.word, rs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wait_own_stores();
         __hip_atomic_store(&b->gensum.word, ((uint64_t)next << 40) | total, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -194,26 +216,25 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
       l2_invalidate();
     }
     st.gen = next;
-    *s_word = total;
+    s_word[0] = total;
+    if (side_total) s_word[1] = ld_relaxed(&b->side_out.word);
   }
   __syncthreads();
-  return *s_word;
+  if (side_total) *side_total = s_word[1];
+  return s_word[0];
 }
 
 // ---------------------------------------------------------------------------
-// Decentralised frontier queue.  Every wavefront of the grid owns a "home"
-// block of kBlk entries in each queue buffer and, when it fills, takes pool
-// blocks from a per-level cursor (one atomic per kBlk claims).  A block's
-// fill count is published with a plain store when the block fills and at the
-// end of the level.  The consumer side reads block counts after the barrier;
-// no tail counter is shared by the whole grid.
+// Decentralised frontier queue.  The next frontier is split into one segment
+// per XCD (capacity n each, so it cannot overflow): a wavefront stages its
+// claims in LDS and moves them to its XCD's segment with one atomic on that
+// segment's tail per flush; at the end of a phase the workgroup's waves flush
+// together with one atomic.  No tail is shared by the whole grid, and the
+// consumer of the next level reads just the segment tails to split the
+// frontier evenly over all workgroups.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBlk = 256;
-
-struct QueueBuf {
-  uint32_t *ent;   // [(home + pool_cap) * kBlk]
-  uint32_t *cnt;   // [home + pool_cap]
-};
+constexpr int kSegments = 8;
+constexpr uint32_t kStage = 256;  // per-wave LDS staging entries
 
 // Visited-bitmap claim: a plain load filters vertices already seen (a stale
 // copy can only say "unvisited", never the reverse, because bits are only
@@ -227,60 +248,43 @@ __device__ __forceinline__ bool claim_vertex(uint32_t *vis, uint32_t w, uint32_t
   return (old & bit) == 0;
 }
 
-// Per-thread state of the edge visitor; blk / tail / claims are
-// wave-uniform.
+// Per-thread state of the edge visitor; out_n / claims are wave-uniform.
 struct Visitor {
   const uint32_t *__restrict__ col;
   uint32_t *level;
   uint32_t *parent;     // may be null
   uint32_t *vis;
-  QueueBuf out;         // next-frontier buffer
-  uint32_t *pool_cursor;
-  uint32_t home_blocks; // index of the first pool block
-  uint32_t pool_cap;    // pool blocks available
-  uint32_t *err;        // set on pool exhaustion (cannot happen when sized as documented)
-  uint32_t blk;         // current output block
-  uint32_t tail;        // entries in it
-  uint32_t claims;      // entries appended this level by this wave
+  uint32_t *seg;        // output segment of this workgroup's XCD
+  uint32_t *seg_tail;   // its tail
+  uint32_t *stage;      // this wave's LDS staging buffer [kStage]
+  uint32_t out_n;       // staged entries
+  uint32_t claims;      // entries appended in the current phase
   uint32_t next_level;
   uint32_t probes, atomics;  // profiling counters (per lane)
+  const int64_t *off;        // row offsets, read for claimed vertices when track_mf
+  bool track_mf;
+  uint64_t mf;               // out-degree sum of vertices claimed by this lane
 
-  __device__ __forceinline__ void begin_level(uint32_t home, uint32_t lvl) {
-    blk = home;
-    tail = 0;
-    claims = 0;
-    next_level = lvl;
+  // Moves this wave's staged entries to the segment (one atomic).
+  __device__ __forceinline__ void flush_wave() {
+    const uint32_t cnt = out_n;
+    uint32_t base = 0;
+    if (lane_id() == 0)
+      base = __hip_atomic_fetch_add(seg_tail, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base = bcast_lane(base, 0);
+    wave_sync();
+    for (uint32_t k = lane_id(); k < cnt; k += kWave) seg[base + k] = stage[k];
+    wave_sync();
+    out_n = 0;
   }
 
-  // Publishes the current block's fill count (call at the end of a phase).
-  __device__ __forceinline__ void publish() {
-    if (lane_id() == 0) out.cnt[blk] = tail;
-  }
-
-  // Appends the lanes flagged in `mask` (lane rank r = mbcnt) to the queue.
+  // Appends the lanes flagged in `mask` (all lanes of the wave call it).
   __device__ __forceinline__ void append(bool mine, uint64_t mask, uint32_t w) {
+    if (mine) stage[out_n + mbcnt(mask)] = w;
     const uint32_t cnt = (uint32_t)__popcll(mask);
-    const uint32_t rank = mbcnt(mask);
-    const uint32_t room = kBlk - tail;
-    if (mine && rank < room) out.ent[(size_t)blk * kBlk + tail + rank] = w;
-    if (cnt < room) {
-      tail += cnt;
-    } else {
-      // current block is full: publish it and continue in a new pool block
-      if (lane_id() == 0) out.cnt[blk] = kBlk;
-      uint32_t j = 0;
-      if (lane_id() == 0)
-        j = __hip_atomic_fetch_add(pool_cursor, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      j = bcast_lane(j, 0);
-      if (j >= pool_cap) {
-        if (lane_id() == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        j = pool_cap - 1;  // keep writes in bounds; the search reports an error
-      }
-      blk = home_blocks + j;
-      tail = cnt - room;
-      if (mine && rank >= room) out.ent[(size_t)blk * kBlk + (rank - room)] = w;
-    }
+    out_n += cnt;
     claims += cnt;
+    if (out_n > kStage - kWave) flush_wave();
   }
 
   // Called by every lane of the wave (wave-uniform control flow); `act`
@@ -293,6 +297,7 @@ struct Visitor {
       if (claimed) {
         level[w] = next_level;
         if (parent) parent[w] = u;
+        if (track_mf) mf += (uint64_t)(off[w + 1] - off[w]);
       }
     }
     const uint64_t mask = __ballot(claimed);

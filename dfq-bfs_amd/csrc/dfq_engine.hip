@@ -62,9 +62,7 @@ struct dfq_graph {
   uint32_t *d_vis = nullptr;
   uint32_t *d_fb = nullptr;
   uint32_t *d_hq = nullptr;
-  uint32_t *d_qent[2] = {nullptr, nullptr};
-  uint32_t *d_qcnt[2] = {nullptr, nullptr};
-  uint32_t home_blocks = 0, pool_cap = 0;  // queue buffers sized for this grid
+  uint32_t *d_qseg[2] = {nullptr, nullptr};  // kSegments x n entries each
   uint64_t *d_prof = nullptr;
   uint32_t prof_levels = 0;
   Control *d_ctl = nullptr;
@@ -93,19 +91,8 @@ const void *kernel_for(uint32_t block) {
   }
 }
 
-void free_queues(dfq_graph *g) {
-  for (int i = 0; i < 2; ++i) {
-    (void)hipFree(g->d_qent[i]);
-    (void)hipFree(g->d_qcnt[i]);
-    g->d_qent[i] = g->d_qcnt[i] = nullptr;
-  }
-  g->home_blocks = g->pool_cap = 0;
-}
-
 // Grid = CUs x co-resident workgroups per CU (a cooperative launch needs
-// every workgroup resident).  Queue buffers hold one home block per
-// wavefront plus ceil(n / kBlk) + home pool blocks, the most one level can
-// take (every wave's pool blocks are full except its last).
+// every workgroup resident).
 int configure(dfq_graph *g, const dfq_config &cfg) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
@@ -115,28 +102,15 @@ int configure(dfq_graph *g, const dfq_config &cfg) {
                                                         (int)cfg.block_size, 0));
   if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
   if (cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)cfg.blocks_per_cu);
-  const int grid = prop.multiProcessorCount * per_cu;
-  const uint32_t home = (uint32_t)grid * (cfg.block_size / kWave);
-  const uint32_t pool = (uint32_t)((g->n + kBlk - 1) / kBlk) + home;
-  if (home != g->home_blocks || pool != g->pool_cap) {
-    free_queues(g);
-    const size_t blocks = (size_t)home + pool;
-    for (int i = 0; i < 2; ++i) {
-      HIP_TRY(hipMalloc(&g->d_qent[i], sizeof(uint32_t) * blocks * kBlk));
-      HIP_TRY(hipMalloc(&g->d_qcnt[i], sizeof(uint32_t) * blocks));
-      HIP_TRY(hipMemset(g->d_qcnt[i], 0, sizeof(uint32_t) * blocks));
-    }
-    g->home_blocks = home;
-    g->pool_cap = pool;
-  }
   g->cfg = cfg;
-  g->grid = grid;
+  g->grid = prop.multiProcessorCount * per_cu;
   g->block = (int)cfg.block_size;
   return DFQ_OK;
 }
 
 void free_device(dfq_graph *g) {
-  free_queues(g);
+  (void)hipFree(g->d_qseg[0]);
+  (void)hipFree(g->d_qseg[1]);
   (void)hipFree(g->d_off);
   (void)hipFree(g->d_col);
   (void)hipFree(g->d_level);
@@ -186,10 +160,8 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.parent = d_parent;
   a.vis = g->d_vis;
   a.fb = g->d_fb;
-  for (int i = 0; i < 2; ++i) {
-    a.q[i].ent = g->d_qent[i];
-    a.q[i].cnt = g->d_qcnt[i];
-  }
+  a.qseg[0] = g->d_qseg[0];
+  a.qseg[1] = g->d_qseg[1];
   a.hq = g->d_hq;
   a.ctl = g->d_ctl;
   a.prof = prof ? g->d_prof : nullptr;
@@ -200,11 +172,10 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.small_max = g->cfg.small_max;
   a.medium_max = g->cfg.medium_max;
   a.large_max = g->cfg.large_max;
-  a.home_blocks = g->home_blocks;
-  a.pool_cap = g->pool_cap;
   a.bottom_up = (g->cfg.direction_opt && g->symmetric) ? 1u : 0u;
-  a.bu_enter = g->cfg.bu_enter_div ? g->n / g->cfg.bu_enter_div : g->n;
+  a.alpha = g->cfg.bu_enter_div;
   a.bu_exit = g->cfg.bu_exit_div ? g->n / g->cfg.bu_exit_div : 0;
+  a.m = g->m;
   return a;
 }
 
@@ -248,8 +219,8 @@ int dfq_config_default(dfq_config *cfg) {
   cfg->medium_max = 2048;
   cfg->large_max = 32768;
   cfg->direction_opt = 0;
-  cfg->bu_enter_div = 64;
-  cfg->bu_exit_div = 512;
+  cfg->bu_enter_div = 14;
+  cfg->bu_exit_div = 24;
   return DFQ_OK;
 }
 
@@ -329,6 +300,8 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   HIP_OR_FAIL(hipMalloc(&g->d_vis, sizeof(uint32_t) * g->n_words));
   HIP_OR_FAIL(hipMalloc(&g->d_fb, sizeof(uint32_t) * ((size_t)g->n_words + 1)));
   HIP_OR_FAIL(hipMalloc(&g->d_hq, sizeof(uint32_t) * n));
+  for (int i = 0; i < 2; ++i)
+    HIP_OR_FAIL(hipMalloc(&g->d_qseg[i], sizeof(uint32_t) * (size_t)kSegments * n));
   HIP_OR_FAIL(hipMalloc(&g->d_ctl, sizeof(Control)));
   HIP_OR_FAIL(hipMemcpy(g->d_off, row_offsets, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
   if (m) HIP_OR_FAIL(hipMemcpy(g->d_col, col_idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice));

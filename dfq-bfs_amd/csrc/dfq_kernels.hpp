@@ -4,8 +4,8 @@
 // design/shared-q/atomic-sa/our-sep/upper-bound/bfs.cu:98-179, launched
 // cooperatively at :283).  Per level (top-down):
 //
-//   phase A  every workgroup reads its share of the frontier blocks written
-//            by the previous level (round-robin over the per-wave blocks, no
+//   phase A  every workgroup takes an equal share of the frontier (the
+//            previous level's per-XCD segments, located from their tails; no
 //            shared cursor) and bins each vertex by out-degree:
 //              deg <= small_max   lane-owned; the wave gathers the edges of
 //                                 its 64 lanes' vertices through a DPP prefix
@@ -39,13 +39,13 @@
 
 namespace dfq {
 
-// Per-level control slot; four rotate: level L appends into slot L%4's pool
-// cursor and huge list, the consumer of level L+1 reads it, and the slot is
-// zeroed during level L+2 for reuse at level L+4.
+// Per-level control slot; four rotate: level L appends into slot L%4's
+// segment tails and huge list, the consumer of level L+1 reads them, and the
+// slot is zeroed during level L+2 for reuse at level L+4.
 struct alignas(256) LevelSlot {
-  uint32_t pool_cursor;  // pool blocks taken by level L's output
-  uint32_t h_count;      // huge-degree vertices in level L's frontier
-  uint32_t pad[62];
+  uint32_t tail[kSegments];  // entries in each segment of level L's output
+  uint32_t h_count;          // huge-degree vertices in level L's frontier
+  uint32_t pad[64 - kSegments - 1];
 };
 
 struct Control {
@@ -77,7 +77,7 @@ struct BfsArgs {
   uint32_t *parent;      // [n] or null
   uint32_t *vis;         // visited bitmap [n_words]
   uint32_t *fb;          // frontier bitmap [n_words + 1] (bottom-up levels)
-  QueueBuf q[2];         // frontier queue buffers
+  uint32_t *qseg[2];     // frontier queues: kSegments segments of n entries each
   uint32_t *hq;          // huge-degree queue [n]
   Control *ctl;
   uint64_t *prof;        // [prof_levels][kProfFields] or null
@@ -88,11 +88,10 @@ struct BfsArgs {
   uint32_t small_max;    // <= 64
   uint32_t medium_max;
   uint32_t large_max;
-  uint32_t home_blocks;  // = gridDim.x * waves per workgroup
-  uint32_t pool_cap;     // pool blocks per buffer
   uint32_t bottom_up;    // direction optimisation allowed (symmetric graph)
-  uint32_t bu_enter;     // switch to bottom-up when the frontier has more vertices
-  uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer
+  uint32_t alpha;        // bottom-up when frontier edges > unexplored edges / alpha
+  uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer vertices
+  uint64_t m;            // CSR entries
 };
 
 template <int BLOCK>
@@ -101,7 +100,6 @@ struct KernelShape {
   static constexpr int kMedCap = 2 * BLOCK;  // medium LDS queue
   static constexpr int kBigCap = BLOCK;      // large LDS queue
   static constexpr int kHTile = 1024;        // huge vertices per phase-H tile
-  static constexpr int kMyBlocks = 64;       // input blocks per pass
 };
 
 template <int BLOCK>
@@ -131,11 +129,15 @@ struct SharedAll {
     SharedA<BLOCK> a;
     SharedH<BLOCK> h;
   } u;
-  uint32_t my_blk[KernelShape<BLOCK>::kMyBlocks];
-  uint32_t my_pre[KernelShape<BLOCK>::kMyBlocks + 1];
+  uint32_t stage[KernelShape<BLOCK>::kWaves][kStage];
+  uint32_t stage_n[KernelShape<BLOCK>::kWaves];
+  uint32_t stage_base[KernelShape<BLOCK>::kWaves];
+  uint32_t seg_pre[kSegments + 1];
+  uint32_t seg;  // output segment of this workgroup (its XCD's index)
   uint32_t wave_sum[KernelShape<BLOCK>::kWaves];
   uint64_t prof[kProfFields];
-  uint64_t bcast;
+  uint64_t bcast[2];
+  uint64_t mf_sum;
   uint32_t med_n, big_n;
 };
 
@@ -205,6 +207,30 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
   }
 }
 
+// Moves every wave's staged entries to the segment with one atomic per
+// workgroup.  Must be reached by all threads of the workgroup.
+template <int BLOCK>
+__device__ __forceinline__ void flush_block(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t wave) {
+  constexpr int W = KernelShape<BLOCK>::kWaves;
+  if (lane_id() == 0) sh.stage_n[wave] = vis.out_n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int i = 0; i < W; ++i) {
+      sh.stage_base[i] = tot;
+      tot += sh.stage_n[i];
+    }
+    uint32_t base = 0;
+    if (tot) base = __hip_atomic_fetch_add(vis.seg_tail, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = 0; i < W; ++i) sh.stage_base[i] += base;
+  }
+  __syncthreads();
+  const uint32_t cnt = vis.out_n, base = sh.stage_base[wave];
+  for (uint32_t k = lane_id(); k < cnt; k += kWave) vis.seg[base + k] = vis.stage[k];
+  vis.out_n = 0;
+  __syncthreads();
+}
+
 // Sum over the workgroup of a wave-uniform value (thread 0 gets the total in
 // the return value, others get garbage).  Ends with a __syncthreads.
 template <int BLOCK>
@@ -216,6 +242,20 @@ __device__ __forceinline__ uint32_t block_sum_uniform(SharedAll<BLOCK> &sh, uint
     for (int i = 0; i < KernelShape<BLOCK>::kWaves; ++i) t += sh.wave_sum[i];
   __syncthreads();
   return t;
+}
+
+// Workgroup sum of the per-lane out-degree sums of claimed vertices (thread
+// 0 gets it); resets the lanes' counters.  Ends with a __syncthreads.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, Visitor &vis) {
+  if (threadIdx.x == 0) sh.mf_sum = 0;
+  __syncthreads();
+  uint64_t v = vis.mf;
+  vis.mf = 0;
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  if (lane_id() == 0 && v) atomicAdd((unsigned long long *)&sh.mf_sum, (unsigned long long)v);
+  __syncthreads();
+  return sh.mf_sum;
 }
 
 // Sum over all lanes of the workgroup of a per-lane value into sh.prof[f]
@@ -239,51 +279,37 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
   __syncthreads();
 }
 
-// Top-down phase A over this workgroup's share of the input blocks.
+// Top-down phase A over this workgroup's equal share of the frontier.
 template <int BLOCK>
 __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
-                                const QueueBuf &in, uint32_t pool_in, LevelSlot &cur,
+                                const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
                                 uint32_t wave, uint64_t &edges) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
-  const uint32_t G = gridDim.x, b = blockIdx.x;
-  const uint32_t H = a.home_blocks;
-  // my blocks: home blocks b + i*G (i < waves) then pool blocks b + i*G
-  const uint32_t n_home = (uint32_t)S::kWaves;
-  const uint32_t n_pool = pool_in > b ? (pool_in - b - 1) / G + 1 : 0;
-  const uint32_t n_mine = n_home + n_pool;
   if (tid == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kSegments; ++i) {
+      sh.seg_pre[i] = t;
+      t += ld_agent(&in_slot.tail[i]);
+    }
+    sh.seg_pre[kSegments] = t;
     sh.med_n = 0;
     sh.big_n = 0;
   }
-  for (uint32_t p0 = 0; p0 < n_mine; p0 += S::kMyBlocks) {
-    const uint32_t np = min((uint32_t)S::kMyBlocks, n_mine - p0);
-    __syncthreads();
-    if (tid < (uint32_t)kWave) {
-      uint32_t c = 0, blk = 0;
-      if (tid < np) {
-        const uint32_t i = p0 + tid;
-        blk = i < n_home ? b + i * G : H + b + (i - n_home) * G;
-        c = in.cnt[blk];
-      }
-      const uint32_t incl = wave_inclusive_scan(c);
-      if (tid < np) {
-        sh.my_blk[tid] = blk;
-        sh.my_pre[tid] = incl - c;
-      }
-      if (tid == kWave - 1) sh.my_pre[np] = incl;
-    }
-    __syncthreads();
-    const uint32_t total = sh.my_pre[np];
-    for (uint32_t base = 0; base < total; base += BLOCK) {
+  __syncthreads();
+  const uint64_t total = sh.seg_pre[kSegments];
+  const uint32_t lo = (uint32_t)(total * blockIdx.x / gridDim.x);
+  const uint32_t hi = (uint32_t)(total * (blockIdx.x + 1) / gridDim.x);
+  {
+    for (uint32_t base = lo; base < hi; base += BLOCK) {
       const uint32_t p = base + tid;
       uint32_t v = 0, deg = 0;
       int64_t start = 0;
-      if (p < total) {
+      if (p < hi) {
         uint32_t j = 0;
-        for (uint32_t step = S::kMyBlocks / 2; step >= 1; step >>= 1)
-          if (j + step < np && sh.my_pre[j + step] <= p) j += step;
-        v = in.ent[(size_t)sh.my_blk[j] * kBlk + (p - sh.my_pre[j])];
+#pragma unroll
+        for (int k = 1; k < kSegments; ++k) j += sh.seg_pre[k] <= p;
+        v = in_seg[(size_t)j * a.n + (p - sh.seg_pre[j])];
         start = a.off[v];
         deg = (uint32_t)(a.off[v + 1] - start);
         edges += deg;
@@ -454,6 +480,7 @@ __device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, 
     if (claimed) {
       a.level[v] = L + 1;
       if (a.parent) a.parent[v] = par;
+      vis.mf += (uint64_t)(a.off[v + 1] - a.off[v]);
     }
     const uint64_t mask = __ballot(claimed);
     if (mask) {
@@ -477,9 +504,23 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   const uint32_t nwaves = gridDim.x * S::kWaves;
   Control *ctl = a.ctl;
 
-  BarrierState bst{0, 0, 0, 0};
-  if (tid == 0) barrier_census(&ctl->bar, bst);
+  BarrierState bst{0, 0, 0, 0, 0};
+  if (tid == 0) {
+    barrier_census(&ctl->bar, bst);
+    sh.seg = bst.rank;
+    sh.bcast[0] = bst.nx;
+  }
   if (tid < kProfFields) sh.prof[tid] = 0;
+  __syncthreads();
+  // one frontier segment per XCD; a device with more XCDs than segments is
+  // refused by every workgroup alike (before any barrier)
+  if (sh.bcast[0] > (uint64_t)kSegments) {
+    if (gtid == 0) {
+      ctl->err = 2;
+      barrier_census_clear(&ctl->bar);
+    }
+    return;
+  }
 
   // ---- initialisation (reference :216 cost init, done on the device) ----
   for (uint32_t i = gtid; i < a.n; i += gstride) {
@@ -488,75 +529,90 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   }
   for (uint32_t w = gtid; w < a.n_words; w += gstride)
     a.vis[w] = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
-  // level-0 frontier: home block of wave 0 holds the source
-  if (lane_id() == 0) a.q[0].cnt[gwave] = (gwave == 0) ? 1u : 0u;
+  // level-0 frontier: the source alone, in segment 0 of buffer 0, whose
+  // tails live in the slot of "level -1" (slot 3)
   if (gtid == 0) {
-    a.q[0].ent[0] = a.source;
+    a.qseg[0][0] = a.source;
     for (int s = 0; s < 4; ++s) {
-      ctl->slot[s].pool_cursor = 0;
+      for (int k = 0; k < kSegments; ++k) ctl->slot[s].tail[k] = 0;
       ctl->slot[s].h_count = 0;
     }
+    ctl->slot[3].tail[0] = 1;
     ctl->err = 0;
   }
-  grid_barrier(&ctl->bar, bst, 0, &sh.bcast);
+  grid_barrier(&ctl->bar, bst, 0, sh.bcast);
 
   Visitor vis;
   vis.col = a.col;
   vis.level = a.level;
   vis.parent = a.parent;
   vis.vis = a.vis;
-  vis.home_blocks = a.home_blocks;
-  vis.pool_cap = a.pool_cap;
-  vis.err = &ctl->err;
+  vis.stage = sh.stage[wave];
+  vis.out_n = 0;
   vis.probes = 0;
   vis.atomics = 0;
 
+  vis.off = a.off;
+  vis.track_mf = a.bottom_up != 0;
+  vis.mf = 0;
+
   uint64_t n_cur = 1, n_prev = 0, reached = 1;
+  // m_f: out-degree sum of the current frontier; m_seen: of every vertex
+  // placed in a frontier so far (m - m_seen = edges left to explore)
+  uint64_t m_f = (uint64_t)(a.off[a.source + 1] - a.off[a.source]);
+  uint64_t m_seen = m_f;
   uint32_t bu_levels = 0;
   bool bottom_up = false;
   for (uint32_t L = 0;; ++L) {
-    const QueueBuf in = a.q[L & 1];
-    LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
+    const uint32_t *in_seg = a.qseg[L & 1];
+    const LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
     LevelSlot &cur = ctl->slot[L & 3];
     if (gtid == 0) {
-      ctl->slot[(L + 2) & 3].pool_cursor = 0;
+      for (int k = 0; k < kSegments; ++k) ctl->slot[(L + 2) & 3].tail[k] = 0;
       ctl->slot[(L + 2) & 3].h_count = 0;
     }
-    // direction choice: identical inputs in every workgroup
+    // direction choice (Beamer et al.): identical inputs in every workgroup
     if (a.bottom_up) {
-      if (!bottom_up && n_cur > a.bu_enter && n_cur > n_prev) bottom_up = true;
+      const uint64_t m_u = a.m > m_seen ? a.m - m_seen : 0;
+      if (!bottom_up && n_cur > n_prev && m_f * a.alpha > m_u) bottom_up = true;
       else if (bottom_up && n_cur < a.bu_exit && n_cur < n_prev) bottom_up = false;
     }
-    vis.out = a.q[(L + 1) & 1];
-    vis.pool_cursor = &cur.pool_cursor;
-    vis.begin_level(gwave, L + 1);
+    vis.seg = a.qseg[(L + 1) & 1] + (size_t)sh.seg * a.n;
+    vis.seg_tail = &cur.tail[sh.seg];
+    vis.next_level = L + 1;
+    vis.claims = 0;
     uint64_t edges = 0;
     uint64_t n_next;
     uint32_t hc = 0;
     if (!bottom_up) {
-      // the cursor can pass pool_cap only when err is already set
-      const uint32_t pool_in = min(ld_agent(&in_slot.pool_cursor), a.pool_cap);
-      topdown_phase_a<BLOCK>(a, sh, vis, in, pool_in, cur, wave, edges);
-      vis.publish();
+      topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, edges);
+      flush_block<BLOCK>(vis, sh, wave);
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
-      n_next = grid_barrier(&ctl->bar, bst, wg_claims, &sh.bcast);
+      uint64_t mf1 = 0;
+      n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      m_f = mf1;
       hc = ld_agent(&cur.h_count);
       if (hc) {
         const uint32_t before = vis.claims;
         topdown_phase_h<BLOCK>(a, sh, vis, hc, wave, edges);
-        vis.publish();
+        flush_block<BLOCK>(vis, sh, wave);
         const uint32_t wg2 = block_sum_uniform<BLOCK>(sh, vis.claims - before, wave);
-        n_next += grid_barrier(&ctl->bar, bst, wg2, &sh.bcast);
+        uint64_t mf2 = 0;
+        n_next += grid_barrier(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
+        m_f += mf2;
       }
     } else {
       ++bu_levels;
       build_frontier_bitmap(a, L, gwave, nwaves);
-      grid_barrier(&ctl->bar, bst, 0, &sh.bcast);
+      grid_barrier(&ctl->bar, bst, 0, sh.bcast);
       bottomup_expand(a, vis, L, gwave, nwaves, edges);
-      vis.publish();
+      flush_block<BLOCK>(vis, sh, wave);
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
-      n_next = grid_barrier(&ctl->bar, bst, wg_claims, &sh.bcast);
+      uint64_t mf1 = 0;
+      n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      m_f = mf1;
     }
+    m_seen += m_f;
     if (a.prof) {
       prof_add<BLOCK>(sh, kProfEdges, edges);
       prof_add<BLOCK>(sh, kProfProbes, vis.probes);

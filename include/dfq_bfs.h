@@ -44,11 +44,12 @@ extern "C" {
  *              deg <= large_max       -> one workgroup per vertex
  *              deg >  large_max       -> whole grid, HBM queue
  * direction_opt = 1 lets a level run bottom-up (every unvisited vertex looks
- * for a parent in the frontier) when the frontier holds more than
- * n / bu_enter_div vertices and is growing, and returns to top-down when a
- * shrinking frontier holds fewer than n / bu_exit_div.  It only takes effect
- * on symmetric graphs (checked when the graph is created); levels are the
- * same either way.  direction_opt = 0 runs every level top-down. */
+ * for a parent in the frontier) when the frontier is growing and its
+ * out-edges exceed the unexplored edges / bu_enter_div (Beamer's alpha,
+ * default 14), and returns to top-down when a shrinking frontier holds fewer
+ * than n / bu_exit_div vertices (beta, default 24).  It only takes effect on
+ * symmetric graphs (checked when the graph is created); levels are the same
+ * either way.  direction_opt = 0 runs every level top-down. */
 typedef struct dfq_config {
   uint32_t block_size;     /* threads per workgroup: 256, 512 or 1024     */
   uint32_t blocks_per_cu;  /* 0 = as many as occupancy allows             */
@@ -56,8 +57,8 @@ typedef struct dfq_config {
   uint32_t medium_max;     /* >= small_max                                 */
   uint32_t large_max;      /* >= medium_max                                */
   uint32_t direction_opt;  /* 0 or 1                                       */
-  uint32_t bu_enter_div;   /* > 0 when direction_opt                       */
-  uint32_t bu_exit_div;    /* > 0 when direction_opt                       */
+  uint32_t bu_enter_div;   /* alpha, > 0 when direction_opt                */
+  uint32_t bu_exit_div;    /* beta, > 0 when direction_opt                 */
   uint32_t reserved;       /* must be zero                                 */
 } dfq_config;
 

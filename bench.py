@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy of BYTES (counter calibration)")
+    ap.add_argument("--then-copy", type=int, default=0, metavar="BYTES",
+                    help="after the benchmark, run the streaming copy of BYTES once (so one "
+                         "counter pass covers the BFS and its calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the search on one "
@@ -202,6 +205,9 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(csr, srcs, levels, args.cpu_budget)
     print(json.dumps(out), flush=True)
+    if args.then_copy:
+        a, b = dfq.DeviceBuffer(args.then_copy), dfq.DeviceBuffer(args.then_copy)
+        log("copy", args.then_copy, "bytes:", round(dfq.stream_copy(b, a, args.then_copy), 4), "ms")
 
 
 if __name__ == "__main__":

@@ -254,6 +254,7 @@ struct Visitor {
   uint32_t *level;
   uint32_t *parent;     // may be null
   uint32_t *vis;
+  const uint32_t *snap; // visited bitmap as of the level's start (subset of vis)
   uint32_t *seg;        // output segment of this workgroup's XCD
   uint32_t *seg_tail;   // its tail
   uint32_t *stage;      // this wave's LDS staging buffer [kStage]
@@ -285,6 +286,51 @@ struct Visitor {
     out_n += cnt;
     claims += cnt;
     if (out_n > kStage - kWave) flush_wave();
+  }
+
+  // Batched form: U edges per lane whose column indices are already loaded.
+  // All U bitmap probes are issued before any result is used, then the
+  // claims (atomic OR for the few that look unvisited) and appends follow,
+  // so one wave keeps U probes in flight instead of one.
+  //
+  // Filter order: the level-start snapshot of the visited bitmap (never
+  // written during the level, so its lines stay in L1/L2), then the live
+  // bitmap for the few neighbours unvisited at level start, then the atomic.
+  // Both loads can only under-report visited vertices, never over-report.
+  template <int U>
+  __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
+                                              const uint32_t (&w)[U]) {
+    uint32_t word[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) word[i] = act[i] ? snap[w[i] >> 5] : 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (!((word[i] >> (w[i] & 31u)) & 1u)) word[i] = vis[w[i] >> 5];
+    uint32_t old[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const uint32_t bit = 1u << (w[i] & 31u);
+      old[i] = bit;
+      if (!(word[i] & bit)) {
+        ++atomics;
+        old[i] = __hip_atomic_fetch_or(&vis[w[i] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      bool claimed = false;
+      if (act[i]) {
+        ++probes;
+        claimed = !(old[i] & (1u << (w[i] & 31u)));
+        if (claimed) {
+          level[w[i]] = next_level;
+          if (parent) parent[w[i]] = u[i];
+          if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+        }
+      }
+      const uint64_t mask = __ballot(claimed);
+      if (mask) append(claimed, mask, w[i]);
+    }
   }
 
   // Called by every lane of the wave (wave-uniform control flow); `act`

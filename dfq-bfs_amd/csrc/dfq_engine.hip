@@ -60,8 +60,10 @@ struct dfq_graph {
   uint32_t *d_level = nullptr;   // outputs for the host-array entry point
   uint32_t *d_parent = nullptr;
   uint32_t *d_vis = nullptr;
+  uint32_t *d_vis_snap = nullptr;
   uint32_t *d_fb = nullptr;
-  uint32_t *d_hq = nullptr;
+  uint2 *d_items = nullptr;
+  uint32_t item_cap = 0;
   uint32_t *d_qseg[2] = {nullptr, nullptr};  // kSegments x n entries each
   uint64_t *d_prof = nullptr;
   uint32_t prof_levels = 0;
@@ -116,8 +118,9 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_level);
   (void)hipFree(g->d_parent);
   (void)hipFree(g->d_vis);
+  (void)hipFree(g->d_vis_snap);
   (void)hipFree(g->d_fb);
-  (void)hipFree(g->d_hq);
+  (void)hipFree(g->d_items);
   (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
@@ -159,10 +162,12 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.level = d_level;
   a.parent = d_parent;
   a.vis = g->d_vis;
+  a.vis_snap = g->d_vis_snap;
   a.fb = g->d_fb;
   a.qseg[0] = g->d_qseg[0];
   a.qseg[1] = g->d_qseg[1];
-  a.hq = g->d_hq;
+  a.items = g->d_items;
+  a.item_cap = g->item_cap;
   a.ctl = g->d_ctl;
   a.prof = prof ? g->d_prof : nullptr;
   a.prof_levels = prof ? g->prof_levels : 0;
@@ -213,11 +218,13 @@ extern "C" {
 int dfq_config_default(dfq_config *cfg) {
   if (!cfg) return DFQ_ERR_INVALID_ARG;
   memset(cfg, 0, sizeof(*cfg));
-  cfg->block_size = 256;
+  // measured on MI355X (RMAT-22, 64 sources): 1024-thread workgroups, one
+  // per CU; rows above 256 entries cut into grid-wide work items
+  cfg->block_size = 1024;
   cfg->blocks_per_cu = 0;
   cfg->small_max = 32;
-  cfg->medium_max = 2048;
-  cfg->large_max = 32768;
+  cfg->medium_max = 256;
+  cfg->large_max = 256;
   cfg->direction_opt = 0;
   cfg->bu_enter_div = 14;
   cfg->bu_exit_div = 24;
@@ -298,8 +305,12 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   HIP_OR_FAIL(hipMalloc(&g->d_level, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_parent, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_vis, sizeof(uint32_t) * g->n_words));
+  HIP_OR_FAIL(hipMalloc(&g->d_vis_snap, sizeof(uint32_t) * g->n_words));
   HIP_OR_FAIL(hipMalloc(&g->d_fb, sizeof(uint32_t) * ((size_t)g->n_words + 1)));
-  HIP_OR_FAIL(hipMalloc(&g->d_hq, sizeof(uint32_t) * n));
+  // one level's items: sum over frontier vertices of ceil(deg / kItem)
+  // <= m / kItem + n; any segment may receive all of them
+  g->item_cap = (uint32_t)std::min<uint64_t>(m / kItem + n, 0xffffffffull);
+  HIP_OR_FAIL(hipMalloc(&g->d_items, sizeof(uint2) * (size_t)kSegments * g->item_cap));
   for (int i = 0; i < 2; ++i)
     HIP_OR_FAIL(hipMalloc(&g->d_qseg[i], sizeof(uint32_t) * (size_t)kSegments * n));
   HIP_OR_FAIL(hipMalloc(&g->d_ctl, sizeof(Control)));

@@ -12,15 +12,15 @@
 //                                 sum so every lane reads one edge per step
 //              deg <= medium_max  LDS queue, one wavefront per vertex
 //              deg <= large_max   LDS queue, one workgroup per vertex
-//              deg >  large_max   HBM queue, expanded by the whole grid
+//              deg >  large_max   cut into 256-edge work items (HBM)
 //            LDS queues are expanded by the workgroup that filled them when
 //            they come within one chunk of full, and drained at the end of
 //            the level (reference :126-169, flush on near-full + drain).
 //   barrier  tree barrier; returns the number of vertices claimed
-//   phase H  only when the level has huge-degree vertices: all workgroups
-//            split their concatenated edge ranges into workgroup-sized
-//            chunks (reference E/F bins :171-176, which serialised F),
-//            then a second barrier.
+//   phase H  only when the level has work items: every wavefront of the grid
+//            takes items round-robin (balanced to one item per wave; the
+//            reference's E/F bins :171-176 serialised F), then a second
+//            barrier.
 //
 // Optionally (symmetric graphs, direction_opt) a level whose frontier is
 // large runs bottom-up instead: every unvisited vertex looks for a neighbour
@@ -43,10 +43,13 @@ namespace dfq {
 // segment tails and huge list, the consumer of level L+1 reads them, and the
 // slot is zeroed during level L+2 for reuse at level L+4.
 struct alignas(256) LevelSlot {
-  uint32_t tail[kSegments];  // entries in each segment of level L's output
-  uint32_t h_count;          // huge-degree vertices in level L's frontier
-  uint32_t pad[64 - kSegments - 1];
+  uint32_t tail[kSegments];       // entries in each segment of level L's output
+  uint32_t item_tail[kSegments];  // work items of level L's grid phase, per segment
+  uint32_t pad[64 - 2 * kSegments];
 };
+
+// Huge-degree frontier vertices are cut into items of kItem CSR entries:
+// (vertex, chunk index).  One item is one wavefront step of kUnroll x 64.
 
 struct Control {
   GridBarrier bar;
@@ -65,10 +68,16 @@ enum ProfField : int {
   kProfProbes,        // visited / frontier-bitmap tests
   kProfAtomics,       // atomic OR attempts on the visited bitmap
   kProfClaims,        // vertices given the next level
-  kProfHuge,          // frontier vertices expanded by the whole grid
+  kProfHuge,          // work items of the grid phase (kItem entries each)
   kProfBottomUp,      // 1 when the level ran bottom-up
-  kProfFields = 8
+  kProfLevelNs,       // wall time of the level (workgroup 0, barrier to barrier)
+  kProfBusyMaxNs,     // slowest workgroup's expansion time (before its barrier arrival)
+  kProfBusySumNs,     // sum over workgroups of the same
+  kProfFields = 10
 };
+
+// 100 MHz constant-rate counter -> ns
+__device__ __forceinline__ uint64_t now_ns() { return __builtin_amdgcn_s_memrealtime() * 10ull; }
 
 struct BfsArgs {
   const int64_t *off;    // row offsets [n+1]
@@ -76,9 +85,11 @@ struct BfsArgs {
   uint32_t *level;       // [n]
   uint32_t *parent;      // [n] or null
   uint32_t *vis;         // visited bitmap [n_words]
+  uint32_t *vis_snap;    // its copy taken at each top-down level's start [n_words]
   uint32_t *fb;          // frontier bitmap [n_words + 1] (bottom-up levels)
   uint32_t *qseg[2];     // frontier queues: kSegments segments of n entries each
-  uint32_t *hq;          // huge-degree queue [n]
+  uint2 *items;          // kSegments segments of item_cap work items
+  uint32_t item_cap;     // items per segment (m / kItem + n bounds one level)
   Control *ctl;
   uint64_t *prof;        // [prof_levels][kProfFields] or null
   uint32_t prof_levels;
@@ -94,12 +105,15 @@ struct BfsArgs {
   uint64_t m;            // CSR entries
 };
 
+// Independent 64-edge steps a wave keeps in flight in the expansions.
+constexpr int kUnroll = 4;
+constexpr uint32_t kItem = kWave * kUnroll;  // CSR entries per work item
+
 template <int BLOCK>
 struct KernelShape {
   static constexpr int kWaves = BLOCK / kWave;
   static constexpr int kMedCap = 2 * BLOCK;  // medium LDS queue
   static constexpr int kBigCap = BLOCK;      // large LDS queue
-  static constexpr int kHTile = 1024;        // huge vertices per phase-H tile
 };
 
 template <int BLOCK>
@@ -116,23 +130,15 @@ struct SharedA {
 };
 
 template <int BLOCK>
-struct SharedH {
-  uint32_t hv[KernelShape<BLOCK>::kHTile];
-  int64_t hs[KernelShape<BLOCK>::kHTile];
-  uint64_t hx[KernelShape<BLOCK>::kHTile];  // exclusive prefix of degrees
-  uint64_t wave_tot[KernelShape<BLOCK>::kWaves];
-};
-
-template <int BLOCK>
 struct SharedAll {
-  union {
+  struct {
     SharedA<BLOCK> a;
-    SharedH<BLOCK> h;
   } u;
   uint32_t stage[KernelShape<BLOCK>::kWaves][kStage];
   uint32_t stage_n[KernelShape<BLOCK>::kWaves];
   uint32_t stage_base[KernelShape<BLOCK>::kWaves];
   uint32_t seg_pre[kSegments + 1];
+  uint32_t item_pre[kSegments + 1];
   uint32_t seg;  // output segment of this workgroup (its XCD's index)
   uint32_t wave_sum[KernelShape<BLOCK>::kWaves];
   uint64_t prof[kProfFields];
@@ -153,21 +159,23 @@ __device__ __forceinline__ void gather_small(Visitor &vis, uint32_t v, int64_t s
   g_v[lane] = v;
   g_s[lane] = start;
   wave_sync();
-  for (uint32_t b = 0; b < total; b += kWave) {
-    const uint32_t e = b + lane;
-    const bool act = e < total;
-    // owner = last lane whose exclusive prefix is <= e; lanes with degree 0
-    // share their prefix with the next lane and are never selected.
-    uint32_t j = 0;
+  for (uint32_t b = 0; b < total; b += kWave * kUnroll) {
+    bool act[kUnroll];
+    uint32_t u[kUnroll], w[kUnroll];
 #pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1)
-      if (g_excl[j + step] <= e) j += step;
-    uint32_t w = 0, u = 0;
-    if (act) {
-      u = g_v[j];
-      w = vis.col[g_s[j] + (int64_t)(e - g_excl[j])];
+    for (int i = 0; i < kUnroll; ++i) {
+      const uint32_t e = b + i * kWave + lane;
+      act[i] = e < total;
+      // owner = last lane whose exclusive prefix is <= e; lanes with degree
+      // 0 share their prefix with the next lane and are never selected.
+      uint32_t j = 0;
+#pragma unroll
+      for (uint32_t step = 32; step >= 1; step >>= 1)
+        if (g_excl[j + step] <= e) j += step;
+      u[i] = g_v[j];
+      w[i] = act[i] ? vis.col[g_s[j] + (int64_t)(e - g_excl[j])] : 0u;
     }
-    vis.visit(act, u, w);
+    vis.visit_batch<kUnroll>(act, u, w);
   }
   wave_sync();
 }
@@ -182,11 +190,17 @@ __device__ __forceinline__ void expand_medium(Visitor &vis, SharedAll<BLOCK> &sh
     const uint32_t u = sh.u.a.med_v[j];
     const uint32_t d = sh.u.a.med_d[j];
     const int64_t s = sh.u.a.med_s[j];
-    for (uint32_t k0 = 0; k0 < d; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      const bool act = k < d;
-      const uint32_t w = act ? vis.col[s + k] : 0u;
-      vis.visit(act, u, w);
+    for (uint32_t k0 = 0; k0 < d; k0 += kWave * kUnroll) {
+      bool act[kUnroll];
+      uint32_t uu[kUnroll], w[kUnroll];
+#pragma unroll
+      for (int i = 0; i < kUnroll; ++i) {
+        const uint32_t k = k0 + i * kWave + lane;
+        act[i] = k < d;
+        uu[i] = u;
+        w[i] = act[i] ? vis.col[s + k] : 0u;
+      }
+      vis.visit_batch<kUnroll>(act, uu, w);
     }
   }
 }
@@ -198,11 +212,17 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
     const uint32_t u = sh.u.a.big_v[j];
     const uint32_t d = sh.u.a.big_d[j];
     const int64_t s = sh.u.a.big_s[j];
-    for (uint32_t k0 = 0; k0 < d; k0 += BLOCK) {
-      const uint32_t k = k0 + threadIdx.x;
-      const bool act = k < d;
-      const uint32_t w = act ? vis.col[s + k] : 0u;
-      vis.visit(act, u, w);
+    for (uint32_t k0 = 0; k0 < d; k0 += BLOCK * kUnroll) {
+      bool act[kUnroll];
+      uint32_t uu[kUnroll], w[kUnroll];
+#pragma unroll
+      for (int i = 0; i < kUnroll; ++i) {
+        const uint32_t k = k0 + i * BLOCK + threadIdx.x;
+        act[i] = k < d;
+        uu[i] = u;
+        w[i] = act[i] ? vis.col[s + k] : 0u;
+      }
+      vis.visit_batch<kUnroll>(act, uu, w);
     }
   }
 }
@@ -271,9 +291,11 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
   __syncthreads();
   if (threadIdx.x < kProfFields) {
     const uint64_t v = sh.prof[threadIdx.x];
-    if (v && L < a.prof_levels)
-      atomicAdd((unsigned long long *)&a.prof[(size_t)L * kProfFields + threadIdx.x],
-                (unsigned long long)v);
+    if (v && L < a.prof_levels) {
+      unsigned long long *dst = (unsigned long long *)&a.prof[(size_t)L * kProfFields + threadIdx.x];
+      if (threadIdx.x == kProfBusyMaxNs) atomicMax(dst, (unsigned long long)v);
+      else atomicAdd(dst, (unsigned long long)v);
+    }
     sh.prof[threadIdx.x] = 0;
   }
   __syncthreads();
@@ -297,22 +319,27 @@ __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
     sh.big_n = 0;
   }
   __syncthreads();
-  const uint64_t total = sh.seg_pre[kSegments];
-  const uint32_t lo = (uint32_t)(total * blockIdx.x / gridDim.x);
-  const uint32_t hi = (uint32_t)(total * (blockIdx.x + 1) / gridDim.x);
+  // 64-entry blocks of the frontier dealt round-robin over the workgroups:
+  // coalesced reads, and a mix of early claims (often high degree) and late
+  // ones in every workgroup
+  const uint32_t total = sh.seg_pre[kSegments];
+  const uint32_t nblk = (total + kWave - 1) / kWave;
+  const uint32_t G = gridDim.x, W = (uint32_t)S::kWaves;
+  const uint32_t mine = nblk > blockIdx.x ? (nblk - blockIdx.x - 1) / G + 1 : 0;
   {
-    for (uint32_t base = lo; base < hi; base += BLOCK) {
-      const uint32_t p = base + tid;
+    for (uint32_t c0 = 0; c0 < mine; c0 += W) {
+      const uint32_t c = c0 + wave;  // this wave's block ordinal within the workgroup
+      const uint32_t p = (blockIdx.x + c * G) * kWave + lane_id();
       uint32_t v = 0, deg = 0;
       int64_t start = 0;
-      if (p < hi) {
+      if (c < mine && p < total) {
         uint32_t j = 0;
 #pragma unroll
         for (int k = 1; k < kSegments; ++k) j += sh.seg_pre[k] <= p;
         v = in_seg[(size_t)j * a.n + (p - sh.seg_pre[j])];
         start = a.off[v];
         deg = (uint32_t)(a.off[v + 1] - start);
-        edges += deg;
+        if (deg <= a.large_max) edges += deg;
       }
       const bool is_small = deg != 0 && deg <= a.small_max;
       const bool is_med = deg > a.small_max && deg <= a.medium_max;
@@ -343,14 +370,19 @@ __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
           sh.u.a.big_s[q] = start;
         }
       }
+      // huge: kItem-entry work items, slots reserved by one atomic per wave
       const uint64_t hm = __ballot(is_huge);
       if (hm) {
+        const uint32_t nit = is_huge ? (deg + kItem - 1) / kItem : 0u;
+        const uint32_t incl = wave_inclusive_scan(nit);
+        const uint32_t tot = bcast_lane(incl, 63);
         uint32_t b0 = 0;
         if (lane_id() == 0)
-          b0 = __hip_atomic_fetch_add(&cur.h_count, (uint32_t)__popcll(hm), __ATOMIC_RELAXED,
+          b0 = __hip_atomic_fetch_add(&cur.item_tail[sh.seg], tot, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
         b0 = bcast_lane(b0, 0);
-        if (is_huge) a.hq[b0 + mbcnt(hm)] = v;
+        uint2 *dst = a.items + (size_t)sh.seg * a.item_cap + b0 + (incl - nit);
+        for (uint32_t k = 0; k < nit; ++k) dst[k] = make_uint2(v, k);
       }
       gather_small<BLOCK>(vis, is_small ? v : 0u, is_small ? start : 0, is_small ? deg : 0u,
                           sh.u.a.g_excl[wave], sh.u.a.g_v[wave], sh.u.a.g_s[wave]);
@@ -377,65 +409,33 @@ __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
   expand_large<BLOCK>(vis, sh, bn);
 }
 
-// Phase H: huge-degree vertices, edges split evenly over the whole grid.
+// Phase H: the level's work items, round-robin over every wavefront of the
+// grid.  `total` items located through sh.item_pre.
 template <int BLOCK>
-__device__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t hc,
-                                uint32_t wave, uint64_t &edges) {
-  using S = KernelShape<BLOCK>;
-  constexpr int HT = S::kHTile;
-  constexpr int RUN = HT / BLOCK;
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t t0 = 0; t0 < hc; t0 += HT) {
-    const uint32_t tn = min((uint32_t)HT, hc - t0);
-    __syncthreads();
-    uint64_t run = 0;
-    for (int r = 0; r < RUN; ++r) {
-      const uint32_t idx = tid * RUN + r;
-      uint32_t hv = 0, d = 0;
-      int64_t s = 0;
-      if (idx < tn) {
-        hv = a.hq[t0 + idx];
-        s = a.off[hv];
-        d = (uint32_t)(a.off[hv + 1] - s);
-      }
-      sh.u.h.hv[idx] = hv;
-      sh.u.h.hs[idx] = s;
-      sh.u.h.hx[idx] = run;
-      run += d;
-    }
-    uint64_t incl = run;
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint64_t t = __shfl_up(incl, d);
-      if (lane_id() >= (uint32_t)d) incl += t;
-    }
-    if (lane_id() == kWave - 1) sh.u.h.wave_tot[wave] = incl;
-    __syncthreads();
-    uint64_t wave_off = 0, tot = 0;
-    for (int w = 0; w < S::kWaves; ++w) {
-      if ((uint32_t)w < wave) wave_off += sh.u.h.wave_tot[w];
-      tot += sh.u.h.wave_tot[w];
-    }
-    const uint64_t excl = wave_off + (incl - run);
-    for (int r = 0; r < RUN; ++r) sh.u.h.hx[tid * RUN + r] += excl;
-    __syncthreads();
-    if (blockIdx.x == 0 && tid == 0) edges += tot;
-    // Entries beyond tn have degree 0 and prefix == tot: never selected.
-    for (uint64_t ch = blockIdx.x; ch * BLOCK < tot; ch += gridDim.x) {
-      const uint64_t e = ch * BLOCK + tid;
-      const bool act = e < tot;
-      uint32_t j = 0;
+__device__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t total,
+                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
+  const uint32_t lane = lane_id();
+  for (uint32_t i = gwave; i < total; i += nwaves) {
+    uint32_t j = 0;
 #pragma unroll
-      for (uint32_t step = HT / 2; step >= 1; step >>= 1)
-        if (sh.u.h.hx[j + step] <= e) j += step;
-      uint32_t w = 0, u = 0;
-      if (act) {
-        u = sh.u.h.hv[j];
-        w = a.col[sh.u.h.hs[j] + (int64_t)(e - sh.u.h.hx[j])];
-      }
-      vis.visit(act, u, w);
+    for (int k = 1; k < kSegments; ++k) j += sh.item_pre[k] <= i;
+    const uint2 it = a.items[(size_t)j * a.item_cap + (i - sh.item_pre[j])];
+    const uint32_t u = it.x;
+    const int64_t s0 = a.off[u], e0 = a.off[u + 1];
+    const int64_t s = s0 + (int64_t)it.y * kItem;
+    const uint32_t len = (uint32_t)min((int64_t)kItem, e0 - s);
+    if (lane == 0) edges += len;
+    bool act[kUnroll];
+    uint32_t uu[kUnroll], w[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const uint32_t e = (uint32_t)k * kWave + lane;
+      act[k] = e < len;
+      uu[k] = u;
+      w[k] = act[k] ? a.col[s + e] : 0u;
     }
+    vis.visit_batch<kUnroll>(act, uu, w);
   }
-  __syncthreads();
 }
 
 // Bottom-up pass 1: frontier bitmap fb = { v : level[v] == L } (one 64-vertex
@@ -466,15 +466,24 @@ __device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, 
     uint32_t par = 0;
     if (v < a.n && !((seen >> lane_id()) & 1ull)) {
       const int64_t s = a.off[v], e = a.off[v + 1];
-      for (int64_t k = s; k < e; ++k) {
-        const uint32_t w = a.col[k];
-        ++vis.probes;
-        ++edges;
-        if ((a.fb[w >> 5] >> (w & 31u)) & 1u) {
-          claimed = true;
-          par = w;
-          break;
-        }
+      // kUnroll neighbours per step: their frontier-bitmap words are fetched
+      // together, and the first neighbour in the frontier (in row order)
+      // becomes the parent
+      for (int64_t k = s; k < e && !claimed; k += kUnroll) {
+        uint32_t w[kUnroll], f[kUnroll];
+#pragma unroll
+        for (int i = 0; i < kUnroll; ++i) w[i] = k + i < e ? a.col[k + i] : 0u;
+#pragma unroll
+        for (int i = 0; i < kUnroll; ++i) f[i] = k + i < e ? a.fb[w[i] >> 5] : 0u;
+#pragma unroll
+        for (int i = kUnroll - 1; i >= 0; --i)
+          if ((f[i] >> (w[i] & 31u)) & 1u) {
+            claimed = true;
+            par = w[i];
+          }
+        const int64_t scanned = min((int64_t)kUnroll, e - k);
+        vis.probes += (uint32_t)scanned;
+        edges += (uint64_t)scanned;
       }
     }
     if (claimed) {
@@ -527,16 +536,20 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
     a.level[i] = (i == a.source) ? 0u : kInf;
     if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
   }
-  for (uint32_t w = gtid; w < a.n_words; w += gstride)
-    a.vis[w] = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
+  for (uint32_t w = gtid; w < a.n_words; w += gstride) {
+    const uint32_t init = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
+    a.vis[w] = init;
+    a.vis_snap[w] = init;
+  }
   // level-0 frontier: the source alone, in segment 0 of buffer 0, whose
   // tails live in the slot of "level -1" (slot 3)
   if (gtid == 0) {
     a.qseg[0][0] = a.source;
-    for (int s = 0; s < 4; ++s) {
-      for (int k = 0; k < kSegments; ++k) ctl->slot[s].tail[k] = 0;
-      ctl->slot[s].h_count = 0;
-    }
+    for (int s = 0; s < 4; ++s)
+      for (int k = 0; k < kSegments; ++k) {
+        ctl->slot[s].tail[k] = 0;
+        ctl->slot[s].item_tail[k] = 0;
+      }
     ctl->slot[3].tail[0] = 1;
     ctl->err = 0;
   }
@@ -547,6 +560,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   vis.level = a.level;
   vis.parent = a.parent;
   vis.vis = a.vis;
+  vis.snap = a.vis_snap;
   vis.stage = sh.stage[wave];
   vis.out_n = 0;
   vis.probes = 0;
@@ -567,10 +581,11 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
     const uint32_t *in_seg = a.qseg[L & 1];
     const LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
     LevelSlot &cur = ctl->slot[L & 3];
-    if (gtid == 0) {
-      for (int k = 0; k < kSegments; ++k) ctl->slot[(L + 2) & 3].tail[k] = 0;
-      ctl->slot[(L + 2) & 3].h_count = 0;
-    }
+    if (gtid == 0)
+      for (int k = 0; k < kSegments; ++k) {
+        ctl->slot[(L + 2) & 3].tail[k] = 0;
+        ctl->slot[(L + 2) & 3].item_tail[k] = 0;
+      }
     // direction choice (Beamer et al.): identical inputs in every workgroup
     if (a.bottom_up) {
       const uint64_t m_u = a.m > m_seen ? a.m - m_seen : 0;
@@ -584,18 +599,36 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
     uint64_t edges = 0;
     uint64_t n_next;
     uint32_t hc = 0;
+    uint64_t t_start = 0, busy = 0;
+    if (a.prof && tid == 0) t_start = now_ns();
     if (!bottom_up) {
+      // refresh the snapshot: words copied while claims go on are still
+      // subsets of the visited set, which is all the filter needs
+      for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
       topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, edges);
       flush_block<BLOCK>(vis, sh, wave);
+      if (a.prof && tid == 0) busy += now_ns() - t_start;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
       uint64_t mf1 = 0;
       n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
       m_f = mf1;
-      hc = ld_agent(&cur.h_count);
+      if (tid == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < kSegments; ++k) {
+          sh.item_pre[k] = t;
+          t += ld_agent(&cur.item_tail[k]);
+        }
+        sh.item_pre[kSegments] = t;
+      }
+      __syncthreads();
+      hc = sh.item_pre[kSegments];
       if (hc) {
         const uint32_t before = vis.claims;
-        topdown_phase_h<BLOCK>(a, sh, vis, hc, wave, edges);
+        uint64_t th = 0;
+        if (a.prof && tid == 0) th = now_ns();
+        topdown_phase_h<BLOCK>(a, sh, vis, hc, gwave, nwaves, edges);
         flush_block<BLOCK>(vis, sh, wave);
+        if (a.prof && tid == 0) busy += now_ns() - th;
         const uint32_t wg2 = block_sum_uniform<BLOCK>(sh, vis.claims - before, wave);
         uint64_t mf2 = 0;
         n_next += grid_barrier(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
@@ -605,8 +638,11 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
       ++bu_levels;
       build_frontier_bitmap(a, L, gwave, nwaves);
       grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+      uint64_t tb = 0;
+      if (a.prof && tid == 0) tb = now_ns();
       bottomup_expand(a, vis, L, gwave, nwaves, edges);
       flush_block<BLOCK>(vis, sh, wave);
+      if (a.prof && tid == 0) busy += now_ns() - tb;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
       uint64_t mf1 = 0;
       n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
@@ -617,11 +653,16 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
       prof_add<BLOCK>(sh, kProfEdges, edges);
       prof_add<BLOCK>(sh, kProfProbes, vis.probes);
       prof_add<BLOCK>(sh, kProfAtomics, vis.atomics);
+      if (tid == 0) {
+        sh.prof[kProfBusyMaxNs] = busy;
+        sh.prof[kProfBusySumNs] = busy;
+      }
       if (tid == 0 && blockIdx.x == 0) {
         sh.prof[kProfFrontier] = n_cur;
         sh.prof[kProfClaims] = n_next;
         sh.prof[kProfHuge] = hc;
         sh.prof[kProfBottomUp] = bottom_up ? 1 : 0;
+        sh.prof[kProfLevelNs] = now_ns() - t_start;
       }
       prof_flush<BLOCK>(a, sh, L);
       vis.probes = 0;

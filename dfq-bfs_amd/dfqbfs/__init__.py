@@ -365,16 +365,17 @@ class DeviceGraph:
     def symmetric(self) -> bool:
         return bool(lib.dfq_graph_is_symmetric(self._h))
 
-    PROFILE_FIELDS = ("frontier", "edges", "probes", "atomics", "claims", "huge", "bottom_up")
+    PROFILE_FIELDS = ("frontier", "edges", "probes", "atomics", "claims", "huge", "bottom_up",
+                      "level_ns", "busy_max_ns", "busy_sum_ns")
 
     def profile(self, source: int, max_levels: int = 64):
         """Per-level counters: list of dicts (levels beyond max_levels dropped)."""
-        rows = np.zeros((max_levels, 8), np.uint64)
+        rows = np.zeros((max_levels, len(self.PROFILE_FIELDS)), np.uint64)
         st = Stats()
         check(lib.dfq_bfs_profile(self._h, source, rows.ctypes.data_as(C.POINTER(C.c_uint64)),
                                   max_levels, C.byref(st)), "dfq_bfs_profile")
         depth = min(st.depth, max_levels)
-        return [dict(zip(self.PROFILE_FIELDS, map(int, rows[i, :7]))) for i in range(depth)], st
+        return [dict(zip(self.PROFILE_FIELDS, map(int, rows[i]))) for i in range(depth)], st
 
     def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
         """BFS into caller-owned device buffers (raw device addresses)."""

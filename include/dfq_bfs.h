@@ -124,12 +124,14 @@ int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
 
 /* BFS with per-level counters (the reference's frontier profiling build,
  * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,140-144,
- * 200-209).  rows[max_levels][8], one row per level:
+ * 200-209).  rows[max_levels][DFQ_PROFILE_FIELDS], one row per level:
  *   0 frontier vertices, 1 CSR entries read, 2 bitmap tests,
  *   3 atomic OR attempts, 4 vertices claimed, 5 grid-expanded vertices,
- *   6 1 if the level ran bottom-up, 7 unused.
+ *   6 1 if the level ran bottom-up, 7 wall ns of the level,
+ *   8 slowest workgroup's expansion ns, 9 sum of the workgroups' ns.
  * Levels beyond max_levels are not recorded.  The counters slow the search;
  * time it with dfq_bfs_device instead. */
+#define DFQ_PROFILE_FIELDS 10
 int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows,
                     uint32_t max_levels, dfq_stats *stats);
 

@@ -46,7 +46,7 @@ def test_strerror_and_config(dfq):
     assert dfq.lib.dfq_strerror(0) == b"ok"
     assert b"range" in dfq.lib.dfq_strerror(dfq.ERR_RANGE)
     c = dfq.default_config()
-    assert c.block_size in (256, 512) and c.small_max <= 64
+    assert c.block_size in (256, 512, 1024) and c.small_max <= 64
     assert c.small_max <= c.medium_max <= c.large_max
 
 

@@ -1,0 +1,410 @@
+// Batched DFQ-BFS: many BFS instances over many resident graphs in one
+// persistent cooperative launch (no counterpart in the reference, which ran
+// one search per process; the per-level machinery follows the plain kernel
+// design/shared-q/atomic-sa/our-sep/upper-bound/bfs.cu:98-179 as rebuilt in
+// dfq_kernels.hpp).
+//
+// Two tiers, chosen per query by the host plan:
+//   grid tier       the whole grid runs one instance at a time (grid_bfs,
+//                   XCD-aware barrier per level) - for large graphs;
+//   workgroup tier  one workgroup runs one instance: visited set and the
+//                   current / next frontier bitmaps live in LDS, claims are
+//                   LDS atomics, levels are separated by __syncthreads and the
+//                   frontier queue is the workgroup's own partition of an HBM
+//                   array.  Workgroups take instances from a shared counter
+//                   (largest first) until none are left.
+#pragma once
+
+#include "dfq_kernels.hpp"
+
+namespace dfq {
+
+// Vertices a workgroup-tier instance may have (three LDS bitmaps of
+// kWgMaxN / 8 bytes each).
+constexpr uint32_t kWgMaxN = 1u << 17;
+constexpr uint32_t kWgWords = kWgMaxN / 32;
+
+struct BatchGraph {
+  uint64_t vbase;      // first row offset of this graph in the concatenated offsets
+  uint64_t ebase;      // first column index of this graph in the concatenated columns
+  uint64_t m;          // CSR entries
+  uint32_t n;          // vertices
+  uint32_t symmetric;  // bottom-up levels allowed
+};
+
+struct BatchQuery {
+  uint64_t lbase;      // first entry of this query's level array in the output
+  uint32_t graph;
+  uint32_t source;
+  uint32_t tier;       // 0 workgroup, 1 grid
+  uint32_t small_max, medium_max, large_max;
+  uint32_t direction_opt, alpha, beta;
+  uint32_t pad;
+};
+
+struct BatchResult {
+  uint32_t depth;
+  uint32_t bu_levels;
+  uint64_t reached;
+};
+
+struct BatchArgs {
+  const int64_t *off;         // concatenated row offsets
+  const uint32_t *col;        // concatenated columns
+  const BatchGraph *graphs;
+  const BatchQuery *queries;  // grid-tier queries first, then workgroup tier
+  BatchResult *results;
+  uint32_t *level;            // concatenated level arrays
+  uint32_t n_queries;
+  uint32_t n_grid;            // queries [0, n_grid) are grid tier
+  uint32_t *work;             // workgroup-tier instance counter
+  uint32_t *wg_queue;         // per-workgroup frontier partitions [grid][2][kWgMaxN]
+  BfsArgs grid;               // scratch of the grid tier (sized for its largest graph)
+};
+
+// ---------------------------------------------------------------------------
+// Workgroup tier
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+struct SharedWg {
+  SharedA<BLOCK> a;                 // bin queues and gather scratch
+  uint32_t vis[kWgWords];           // visited
+  uint32_t fb[2][kWgWords];         // frontier bitmaps (current, next) by level parity
+  uint32_t wave_sum[BLOCK / kWave];
+  uint64_t mf;                      // out-degree sum of the next frontier
+  uint32_t out_n;                   // next frontier size
+  uint32_t med_n, big_n;
+  uint32_t query;
+};
+
+template <int BLOCK>
+union SharedBatch {
+  SharedAll<BLOCK> grid;
+  SharedWg<BLOCK> wg;
+};
+
+// Edge visitor of the workgroup tier (LDS visited set, LDS queue tail).
+struct WgVisitor {
+  const uint32_t *__restrict__ col;
+  const int64_t *off;
+  uint32_t *level;
+  uint32_t *vis;       // LDS
+  uint32_t *fb_next;   // LDS
+  uint32_t *q_out;     // HBM partition
+  uint32_t *out_n;     // LDS
+  uint32_t next_level;
+  bool track_mf;
+  uint64_t mf;
+
+  __device__ __forceinline__ void append(bool mine, uint64_t mask, uint32_t w) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(out_n, (uint32_t)__popcll(mask));
+    base = bcast_lane(base, 0);
+    if (mine) q_out[base + mbcnt(mask)] = w;
+  }
+
+  template <int U>
+  __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&w)[U]) {
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      bool claimed = false;
+      if (act[i]) {
+        const uint32_t bit = 1u << (w[i] & 31u);
+        if (!(vis[w[i] >> 5] & bit)) claimed = !(atomicOr(&vis[w[i] >> 5], bit) & bit);
+        if (claimed) {
+          level[w[i]] = next_level;
+          atomicOr(&fb_next[w[i] >> 5], bit);
+          if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+        }
+      }
+      const uint64_t mask = __ballot(claimed);
+      if (mask) append(claimed, mask, w[i]);
+    }
+  }
+};
+
+template <int BLOCK>
+__device__ __forceinline__ void wg_gather_small(WgVisitor &vis, uint32_t v, int64_t start, uint32_t deg,
+                                                uint32_t *g_excl, int64_t *g_s) {
+  const uint32_t lane = lane_id();
+  const uint32_t incl = wave_inclusive_scan(deg);
+  const uint32_t total = bcast_lane(incl, 63);
+  if (total == 0) return;
+  g_excl[lane] = incl - deg;
+  g_s[lane] = start;
+  (void)v;
+  wave_sync();
+  for (uint32_t b = 0; b < total; b += kWave * kUnroll) {
+    bool act[kUnroll];
+    uint32_t w[kUnroll];
+#pragma unroll
+    for (int i = 0; i < kUnroll; ++i) {
+      const uint32_t e = b + i * kWave + lane;
+      act[i] = e < total;
+      uint32_t j = 0;
+#pragma unroll
+      for (uint32_t step = 32; step >= 1; step >>= 1)
+        if (g_excl[j + step] <= e) j += step;
+      w[i] = act[i] ? vis.col[g_s[j] + (int64_t)(e - g_excl[j])] : 0u;
+    }
+    vis.visit_batch<kUnroll>(act, w);
+  }
+  wave_sync();
+}
+
+// Expands `cnt` rows from LDS queue arrays, `width` lanes per row
+// (64 = one wave per vertex, BLOCK = the whole workgroup per vertex).
+template <int BLOCK, int WIDTH>
+__device__ __forceinline__ void wg_expand_rows(WgVisitor &vis, const uint32_t *qd, const int64_t *qs,
+                                               uint32_t cnt) {
+  const uint32_t lane = WIDTH == BLOCK ? threadIdx.x : lane_id();
+  const uint32_t first = WIDTH == BLOCK ? 0 : threadIdx.x / kWave;
+  const uint32_t stride = WIDTH == BLOCK ? 1 : BLOCK / kWave;
+  for (uint32_t j = first; j < cnt; j += stride) {
+    const uint32_t d = qd[j];
+    const int64_t s = qs[j];
+    for (uint32_t k0 = 0; k0 < d; k0 += WIDTH * kUnroll) {
+      bool act[kUnroll];
+      uint32_t w[kUnroll];
+#pragma unroll
+      for (int i = 0; i < kUnroll; ++i) {
+        const uint32_t k = k0 + i * WIDTH + lane;
+        act[i] = k < d;
+        w[i] = act[i] ? vis.col[s + k] : 0u;
+      }
+      vis.visit_batch<kUnroll>(act, w);
+    }
+  }
+}
+
+// One BFS by one workgroup.  off / col point at the instance's graph.
+template <int BLOCK>
+__device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *col, uint32_t n,
+                       bool symmetric, const BatchQuery &q, uint32_t *level, uint32_t *qbuf,
+                       BatchResult *res) {
+  using S = KernelShape<BLOCK>;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / kWave;
+  const uint32_t words = (n + 31) / 32;
+  const uint32_t src = q.source;
+  for (uint32_t i = tid; i < n; i += BLOCK) level[i] = (i == src) ? 0u : kInf;
+  for (uint32_t w = tid; w < words; w += BLOCK) {
+    const uint32_t init = (w == (src >> 5)) ? (1u << (src & 31u)) : 0u;
+    sh.vis[w] = init;
+    sh.fb[0][w] = init;
+    sh.fb[1][w] = 0;
+  }
+  if (tid == 0) {
+    qbuf[0] = src;
+    sh.out_n = 0;
+    sh.mf = 0;
+  }
+  __syncthreads();
+
+  WgVisitor vis;
+  vis.col = col;
+  vis.off = off;
+  vis.level = level;
+  vis.vis = sh.vis;
+  vis.out_n = &sh.out_n;
+  vis.track_mf = q.direction_opt && symmetric;
+
+  uint32_t n_cur = 1, n_prev = 0, bu_levels = 0;
+  uint64_t reached = 1;
+  uint64_t m_f = (uint64_t)(off[src + 1] - off[src]), m_seen = m_f;
+  bool bottom_up = false;
+  const uint32_t bu_exit = q.beta ? n / q.beta : 0;
+  uint32_t L = 0;
+  for (;; ++L) {
+    const uint32_t *q_in = qbuf + (size_t)(L & 1) * kWgMaxN;
+    uint32_t *fb_cur = sh.fb[L & 1];
+    uint32_t *fb_next = sh.fb[(L + 1) & 1];
+    vis.q_out = qbuf + (size_t)((L + 1) & 1) * kWgMaxN;
+    vis.fb_next = fb_next;
+    vis.next_level = L + 1;
+    vis.mf = 0;
+    if (vis.track_mf) {
+      const uint64_t m_u = m_seen < (uint64_t)off[n] ? (uint64_t)off[n] - m_seen : 0;
+      if (!bottom_up && n_cur > n_prev && m_f * q.alpha > m_u) bottom_up = true;
+      else if (bottom_up && n_cur < bu_exit && n_cur < n_prev) bottom_up = false;
+    }
+    if (!bottom_up) {
+      if (tid == 0) {
+        sh.med_n = 0;
+        sh.big_n = 0;
+      }
+      __syncthreads();
+      for (uint32_t base = 0; base < n_cur; base += BLOCK) {
+        const uint32_t p = base + tid;
+        uint32_t v = 0, deg = 0;
+        int64_t start = 0;
+        if (p < n_cur) {
+          v = q_in[p];
+          start = off[v];
+          deg = (uint32_t)(off[v + 1] - start);
+        }
+        const bool is_small = deg != 0 && deg <= q.small_max;
+        const bool is_med = deg > q.small_max && deg <= q.medium_max;
+        const bool is_big = deg > q.medium_max;
+        const uint64_t mm = __ballot(is_med);
+        if (mm) {
+          uint32_t b0 = 0;
+          if (lane_id() == 0) b0 = atomicAdd(&sh.med_n, (uint32_t)__popcll(mm));
+          b0 = bcast_lane(b0, 0);
+          if (is_med) {
+            sh.a.med_d[b0 + mbcnt(mm)] = deg;
+            sh.a.med_s[b0 + mbcnt(mm)] = start;
+          }
+        }
+        const uint64_t bm = __ballot(is_big);
+        if (bm) {
+          uint32_t b0 = 0;
+          if (lane_id() == 0) b0 = atomicAdd(&sh.big_n, (uint32_t)__popcll(bm));
+          b0 = bcast_lane(b0, 0);
+          if (is_big) {
+            sh.a.big_d[b0 + mbcnt(bm)] = deg;
+            sh.a.big_s[b0 + mbcnt(bm)] = start;
+          }
+        }
+        wg_gather_small<BLOCK>(vis, v, is_small ? start : 0, is_small ? deg : 0u, sh.a.g_excl[wave],
+                               sh.a.g_s[wave]);
+        __syncthreads();
+        const uint32_t mn = sh.med_n, bn = sh.big_n;
+        __syncthreads();
+        if (mn > (uint32_t)(S::kMedCap - BLOCK)) {
+          wg_expand_rows<BLOCK, kWave>(vis, sh.a.med_d, sh.a.med_s, mn);
+          __syncthreads();
+          if (tid == 0) sh.med_n = 0;
+        }
+        if (bn > (uint32_t)(S::kBigCap - BLOCK)) {
+          wg_expand_rows<BLOCK, BLOCK>(vis, sh.a.big_d, sh.a.big_s, bn);
+          __syncthreads();
+          if (tid == 0) sh.big_n = 0;
+        }
+        __syncthreads();
+      }
+      const uint32_t mn = sh.med_n, bn = sh.big_n;
+      wg_expand_rows<BLOCK, kWave>(vis, sh.a.med_d, sh.a.med_s, mn);
+      wg_expand_rows<BLOCK, BLOCK>(vis, sh.a.big_d, sh.a.big_s, bn);
+    } else {
+      ++bu_levels;
+      // every unvisited vertex looks for a neighbour in the current
+      // frontier; 64-vertex groups per wave own their bitmap words
+      const uint32_t groups = (n + 63) / 64;
+      for (uint32_t g = wave; g < groups; g += BLOCK / kWave) {
+        const uint32_t v = g * 64 + lane_id();
+        const uint32_t w0 = sh.vis[2 * g];
+        const uint32_t w1 = 2 * g + 1 < words ? sh.vis[2 * g + 1] : 0xffffffffu;
+        const uint64_t seen = ((uint64_t)w1 << 32) | w0;
+        bool claimed = false;
+        if (v < n && !((seen >> lane_id()) & 1ull)) {
+          const int64_t s = off[v], e = off[v + 1];
+          for (int64_t k = s; k < e && !claimed; k += kUnroll) {
+            uint32_t w[kUnroll];
+#pragma unroll
+            for (int i = 0; i < kUnroll; ++i) w[i] = k + i < e ? col[k + i] : 0xffffffffu;
+#pragma unroll
+            for (int i = 0; i < kUnroll; ++i)
+              if (w[i] != 0xffffffffu && ((fb_cur[w[i] >> 5] >> (w[i] & 31u)) & 1u)) claimed = true;
+          }
+          if (claimed) {
+            level[v] = L + 1;
+            vis.mf += (uint64_t)(e - s);
+          }
+        }
+        const uint64_t mask = __ballot(claimed);
+        if (mask) {
+          if (lane_id() == 0) {
+            sh.vis[2 * g] = w0 | (uint32_t)mask;
+            fb_next[2 * g] |= (uint32_t)mask;
+          }
+          if (lane_id() == 1 && 2 * g + 1 < words) {
+            sh.vis[2 * g + 1] = w1 | (uint32_t)(mask >> 32);
+            fb_next[2 * g + 1] |= (uint32_t)(mask >> 32);
+          }
+          vis.append(claimed, mask, v);
+        }
+      }
+    }
+    // next frontier size and out-degree sum
+    uint64_t mfl = vis.mf;
+    for (int d = 32; d >= 1; d >>= 1) mfl += __shfl_xor(mfl, d);
+    if (lane_id() == 0 && mfl) atomicAdd((unsigned long long *)&sh.mf, (unsigned long long)mfl);
+    __syncthreads();
+    const uint32_t n_next = sh.out_n;
+    const uint64_t mf_next = sh.mf;
+    __syncthreads();
+    // clear the bitmap that becomes the next level's "next"
+    for (uint32_t w = tid; w < words; w += BLOCK) fb_cur[w] = 0;
+    if (tid == 0) {
+      sh.out_n = 0;
+      sh.mf = 0;
+    }
+    __syncthreads();
+    reached += n_next;
+    if (n_next == 0) break;
+    m_f = mf_next;
+    m_seen += mf_next;
+    n_prev = n_cur;
+    n_cur = n_next;
+  }
+  if (tid == 0) {
+    res->depth = L + 1;
+    res->reached = reached;
+    res->bu_levels = bu_levels;
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
+  __shared__ SharedBatch<BLOCK> sh;
+  BarrierState bst{0, 0, 0, 0, 0};
+  if (!kernel_prologue<BLOCK>(b.grid.ctl, sh.grid, bst)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *b.work = 0;
+  grid_barrier(&b.grid.ctl->bar, bst, 0, sh.grid.bcast);
+  if (blockIdx.x == 0 && threadIdx.x == 0) barrier_census_clear(&b.grid.ctl->bar);
+
+  // grid tier: one instance at a time over the whole grid
+  for (uint32_t qi = 0; qi < b.n_grid; ++qi) {
+    const BatchQuery q = b.queries[qi];
+    const BatchGraph gr = b.graphs[q.graph];
+    BfsArgs a = b.grid;
+    a.off = b.off + gr.vbase;
+    a.col = b.col + gr.ebase;
+    a.level = b.level + q.lbase;
+    a.parent = nullptr;
+    a.n = gr.n;
+    a.n_words = (gr.n + 31) / 32;
+    a.m = gr.m;
+    a.source = q.source;
+    a.small_max = q.small_max;
+    a.medium_max = q.medium_max;
+    a.large_max = q.large_max;
+    a.bottom_up = (q.direction_opt && gr.symmetric) ? 1u : 0u;
+    a.alpha = q.alpha;
+    a.bu_exit = q.beta ? gr.n / q.beta : 0;
+    grid_bfs<BLOCK>(a, sh.grid, bst);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      b.results[qi].depth = b.grid.ctl->depth;
+      b.results[qi].reached = b.grid.ctl->reached;
+      b.results[qi].bu_levels = b.grid.ctl->bu_levels;
+    }
+  }
+
+  // workgroup tier: instances from the shared counter, one per workgroup
+  uint32_t *qbuf = b.wg_queue + (size_t)blockIdx.x * 2 * kWgMaxN;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      sh.wg.query = b.n_grid + __hip_atomic_fetch_add(b.work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t qi = sh.wg.query;
+    if (qi >= b.n_queries) break;
+    const BatchQuery q = b.queries[qi];
+    const BatchGraph gr = b.graphs[q.graph];
+    wg_bfs<BLOCK>(sh.wg, b.off + gr.vbase, b.col + gr.ebase, gr.n, gr.symmetric != 0, q,
+                  b.level + q.lbase, qbuf, &b.results[qi]);
+  }
+}
+
+}  // namespace dfq

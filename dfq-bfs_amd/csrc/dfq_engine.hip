@@ -128,33 +128,6 @@ void free_device(dfq_graph *g) {
   if (g->stream) (void)hipStreamDestroy(g->stream);
 }
 
-// Bottom-up levels scan a vertex's out-edges as its in-edges, which is only
-// valid when every u->v has a v->u.  Requires sorted rows (binary search);
-// a graph with unsorted rows is treated as directed.
-bool is_symmetric(const int64_t *off, const uint32_t *col, uint32_t n) {
-  std::atomic<bool> ok{true};
-  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
-    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
-      for (int64_t k = off[u] + 1; k < off[u + 1]; ++k)
-        if (col[k - 1] > col[k]) {
-          ok = false;
-          return;
-        }
-  });
-  if (!ok) return false;
-  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
-    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
-      for (int64_t k = off[u]; k < off[u + 1]; ++k) {
-        const uint32_t v = col[k];
-        if (!std::binary_search(col + off[v], col + off[v + 1], (uint32_t)u)) {
-          ok = false;
-          return;
-        }
-      }
-  });
-  return ok;
-}
-
 BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, bool prof) {
   BfsArgs a;
   a.off = g->d_off;

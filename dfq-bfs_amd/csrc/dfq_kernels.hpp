@@ -500,11 +500,12 @@ __device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, 
   }
 }
 
+// One BFS over the whole grid (every workgroup must call it with the same
+// arguments): initialisation, then the level loop until a level claims no
+// vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
+__device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, BarrierState &bst) {
   using S = KernelShape<BLOCK>;
-  __shared__ SharedAll<BLOCK> sh;
-
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
   const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -512,24 +513,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   const uint32_t gwave = blockIdx.x * S::kWaves + wave;
   const uint32_t nwaves = gridDim.x * S::kWaves;
   Control *ctl = a.ctl;
-
-  BarrierState bst{0, 0, 0, 0, 0};
-  if (tid == 0) {
-    barrier_census(&ctl->bar, bst);
-    sh.seg = bst.rank;
-    sh.bcast[0] = bst.nx;
-  }
   if (tid < kProfFields) sh.prof[tid] = 0;
-  __syncthreads();
-  // one frontier segment per XCD; a device with more XCDs than segments is
-  // refused by every workgroup alike (before any barrier)
-  if (sh.bcast[0] > (uint64_t)kSegments) {
-    if (gtid == 0) {
-      ctl->err = 2;
-      barrier_census_clear(&ctl->bar);
-    }
-    return;
-  }
 
   // ---- initialisation (reference :216 cost init, done on the device) ----
   for (uint32_t i = gtid; i < a.n; i += gstride) {
@@ -674,14 +658,42 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
         ctl->depth = L + 1;
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
-        // every workgroup read the census before the first barrier
-        barrier_census_clear(&ctl->bar);
       }
       break;
     }
     n_prev = n_cur;
     n_cur = n_next;
   }
+}
+
+// Census of workgroups per XCD; false (in every workgroup alike) when the
+// device has more XCDs than frontier segments.
+template <int BLOCK>
+__device__ bool kernel_prologue(Control *ctl, SharedAll<BLOCK> &sh, BarrierState &bst) {
+  if (threadIdx.x == 0) {
+    barrier_census(&ctl->bar, bst);
+    sh.seg = bst.rank;
+    sh.bcast[0] = bst.nx;
+  }
+  __syncthreads();
+  if (sh.bcast[0] > (uint64_t)kSegments) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->err = 2;
+      barrier_census_clear(&ctl->bar);
+    }
+    return false;
+  }
+  return true;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
+  __shared__ SharedAll<BLOCK> sh;
+  BarrierState bst{0, 0, 0, 0, 0};
+  if (!kernel_prologue<BLOCK>(a.ctl, sh, bst)) return;
+  grid_bfs<BLOCK>(a, sh, bst);
+  // every workgroup read the census before the search's first barrier
+  if (blockIdx.x == 0 && threadIdx.x == 0) barrier_census_clear(&a.ctl->bar);
 }
 
 }  // namespace dfq

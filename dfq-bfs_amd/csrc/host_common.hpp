@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -72,6 +73,33 @@ void parallel_for(uint64_t n, F fn, unsigned threads = 0) {
     pool.emplace_back([=, &fn] { fn(b, e, t); });
   }
   for (auto &th : pool) th.join();
+}
+
+// Bottom-up levels scan a vertex's out-edges as its in-edges, which is only
+// valid when every u->v has a v->u.  Requires sorted rows (binary search);
+// a graph with unsorted rows is treated as directed.
+inline bool is_symmetric(const int64_t *off, const uint32_t *col, uint32_t n) {
+  std::atomic<bool> ok{true};
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
+      for (int64_t k = off[u] + 1; k < off[u + 1]; ++k)
+        if (col[k - 1] > col[k]) {
+          ok = false;
+          return;
+        }
+  });
+  if (!ok) return false;
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t u = b; u < e && ok.load(std::memory_order_relaxed); ++u)
+      for (int64_t k = off[u]; k < off[u + 1]; ++k) {
+        const uint32_t v = col[k];
+        if (!std::binary_search(col + off[v], col + off[v + 1], (uint32_t)u)) {
+          ok = false;
+          return;
+        }
+      }
+  });
+  return ok;
 }
 
 }  // namespace dfq

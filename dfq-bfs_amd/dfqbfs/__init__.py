@@ -429,3 +429,123 @@ def stream_copy(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> float:
     check(lib.dfq_stream_copy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, C.byref(ms)),
           "dfq_stream_copy")
     return ms.value
+
+
+# ---- batched searches (include/dfq_batch.h) ---------------------------------
+
+TIER_WORKGROUP = 0
+TIER_GRID = 1
+WG_MAX_VERTICES = 131072
+
+
+class QueryPlan(C.Structure):
+    _fields_ = [("tier", C.c_uint32), ("small_max", C.c_uint32), ("medium_max", C.c_uint32),
+                ("large_max", C.c_uint32), ("direction_opt", C.c_uint32), ("alpha", C.c_uint32),
+                ("beta", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("queries", C.c_uint32), ("grid_queries", C.c_uint32),
+                ("wg_queries", C.c_uint32), ("graph_replay", C.c_uint32), ("block_size", C.c_uint32),
+                ("grid_blocks", C.c_uint32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("bucket", C.c_uint32), ("queries", C.c_uint32), ("plan", QueryPlan),
+                ("cost_us", C.c_double), ("sub_batch_ms", C.c_double)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("entries", C.c_uint32), ("block_size", C.c_uint32), ("candidates", C.c_uint32),
+                ("reserved", C.c_uint32), ("entry", TuneEntry * 32)]
+
+
+_sig("dfq_batch_create", C.c_int, C.c_uint32, _u32p, C.POINTER(C.c_uint64), _i64p, _u32p,
+     C.POINTER(_P))
+_sig("dfq_batch_destroy", None, _P)
+_sig("dfq_batch_set_queries", C.c_int, _P, C.c_uint32, _u32p, _u32p)
+_sig("dfq_batch_set_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
+_sig("dfq_batch_get_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
+_sig("dfq_batch_set_block_size", C.c_int, _P, C.c_uint32)
+_sig("dfq_batch_run", C.c_int, _P, C.POINTER(BatchStats))
+_sig("dfq_batch_levels", C.c_int, _P, C.c_uint32, _u32p)
+_sig("dfq_batch_result", C.c_int, _P, C.c_uint32, _u32p, C.POINTER(C.c_uint64))
+_sig("dfq_batch_autotune", C.c_int, _P, C.c_uint32, C.POINTER(TuneReport))
+
+
+class Batch:
+    """Many resident graphs; many (graph, source) queries per launch."""
+
+    def __init__(self, graphs):
+        self.graphs = list(graphs)
+        n = np.array([g.n for g in self.graphs], np.uint32)
+        m = np.array([g.m for g in self.graphs], np.uint64)
+        ro = np.concatenate([g.row_offsets for g in self.graphs]).astype(np.int64)
+        ci = (np.concatenate([g.col_idx for g in self.graphs]).astype(np.uint32)
+              if m.sum() else np.zeros(1, np.uint32))
+        self.n = n
+        h = _P()
+        check(lib.dfq_batch_create(len(self.graphs), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                   _i64(ro), _u32(ci), C.byref(h)), "dfq_batch_create")
+        self._h = h
+        self.n_queries = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.dfq_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_queries(self, graph_ids, sources):
+        gi = np.ascontiguousarray(graph_ids, np.uint32)
+        so = np.ascontiguousarray(sources, np.uint32)
+        check(lib.dfq_batch_set_queries(self._h, gi.size, _u32(gi), _u32(so)), "dfq_batch_set_queries")
+        self.q_graph, self.q_source, self.n_queries = gi.copy(), so.copy(), gi.size
+
+    def plan(self, bucket: int) -> QueryPlan:
+        p = QueryPlan()
+        check(lib.dfq_batch_get_plan(self._h, bucket, C.byref(p)), "dfq_batch_get_plan")
+        return p
+
+    def set_plan(self, bucket: int, **kw):
+        p = self.plan(bucket)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        check(lib.dfq_batch_set_plan(self._h, bucket, C.byref(p)), "dfq_batch_set_plan")
+
+    def set_block_size(self, block: int):
+        check(lib.dfq_batch_set_block_size(self._h, block), "dfq_batch_set_block_size")
+
+    def run(self) -> BatchStats:
+        st = BatchStats()
+        check(lib.dfq_batch_run(self._h, C.byref(st)), "dfq_batch_run")
+        return st
+
+    def levels(self, q: int) -> np.ndarray:
+        out = np.empty(int(self.n[self.q_graph[q]]), np.uint32)
+        check(lib.dfq_batch_levels(self._h, q, _u32(out)), "dfq_batch_levels")
+        return out
+
+    def result(self, q: int):
+        d, r = C.c_uint32(), C.c_uint64()
+        check(lib.dfq_batch_result(self._h, q, C.byref(d), C.byref(r)), "dfq_batch_result")
+        return d.value, r.value
+
+    def autotune(self, repeats: int = 1):
+        rep = TuneReport()
+        check(lib.dfq_batch_autotune(self._h, repeats, C.byref(rep)), "dfq_batch_autotune")
+        return {"block_size": rep.block_size, "candidates": rep.candidates,
+                "buckets": [dict(bucket=e.bucket, queries=e.queries, plan=e.plan.as_dict(),
+                                 cost_us=e.cost_us, sub_batch_ms=e.sub_batch_ms)
+                            for e in rep.entry[:rep.entries]]}

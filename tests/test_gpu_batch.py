@@ -1,0 +1,111 @@
+"""Batched searches (include/dfq_batch.h): every query's levels equal the
+oracle's and the plain engine's, whatever tier and plan runs it."""
+import numpy as np
+import pytest
+
+from conftest import UNREACHED
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_graphs(dfq):
+    gs = [dfq.Csr.rmat(s, 8, seed=100 + s) for s in (6, 8, 10, 11, 12, 14, 16)]
+    gs.append(dfq.Csr.amazon_like(5000, seed=9))           # directed
+    gs.append(dfq.Csr.grid2d(120, 90, 0.05, seed=3))        # long diameter
+    gs.append(dfq.Csr.rmat(17, 8, seed=117))                # 131072 vertices: largest wg-tier graph
+    gs.append(dfq.Csr.rmat(18, 8, seed=118))                # grid tier only
+    return gs
+
+
+def queries(gs, per=3):
+    gi, so = [], []
+    for i, g in enumerate(gs):
+        for s in g.pick_sources(per, seed=i):
+            gi.append(i)
+            so.append(int(s))
+    return np.array(gi), np.array(so)
+
+
+@pytest.fixture(scope="module")
+def batch_setup(dfq, oracle):
+    gs = make_graphs(dfq)
+    gi, so = queries(gs)
+    expect = [oracle.bfs(gs[g].row_offsets, gs[g].col_idx, int(s)) for g, s in zip(gi, so)]
+    return gs, gi, so, expect
+
+
+def check_all(b, gi, so, expect):
+    for q in range(len(gi)):
+        lv = b.levels(q)
+        np.testing.assert_array_equal(lv, expect[q], err_msg=f"query {q} graph {gi[q]} source {so[q]}")
+        depth, reached = b.result(q)
+        e = expect[q]
+        assert reached == int((e != UNREACHED).sum())
+        assert depth == int(e[e != UNREACHED].max()) + 1
+
+
+@pytest.mark.parametrize("mode", ["default", "all_wg", "all_grid", "no_direction", "block512"])
+def test_batch_levels_match_oracle(dfq, batch_setup, mode):
+    gs, gi, so, expect = batch_setup
+    with dfq.Batch(gs) as b:
+        b.set_queries(gi, so)
+        for k in range(32):
+            if mode == "all_wg" and (1 << k) <= dfq.WG_MAX_VERTICES:
+                b.set_plan(k, tier=dfq.TIER_WORKGROUP)
+            if mode == "all_grid":
+                b.set_plan(k, tier=dfq.TIER_GRID)
+            if mode == "no_direction":
+                b.set_plan(k, direction_opt=0)
+        if mode == "block512":
+            b.set_block_size(512)
+        for rep in range(3):  # the first run captures the graph, the others replay it
+            say("batch", mode, "run", rep)
+            st = b.run()
+            assert st.queries == len(gi)
+            check_all(b, gi, so, expect)
+        if mode == "all_grid":
+            assert st.wg_queries == 0
+        if mode == "all_wg":
+            assert st.grid_queries == 3  # only the scale-18 graph
+
+
+def test_batch_matches_plain_engine(dfq, batch_setup):
+    gs, gi, so, expect = batch_setup
+    with dfq.Batch(gs) as b:
+        b.set_queries(gi, so)
+        b.run()
+        for q in range(0, len(gi), 4):
+            with dfq.DeviceGraph(gs[gi[q]]) as g:
+                np.testing.assert_array_equal(g.bfs(int(so[q])).level, b.levels(q))
+
+
+def test_batch_autotune_keeps_results(dfq, batch_setup):
+    gs, gi, so, expect = batch_setup
+    with dfq.Batch(gs) as b:
+        b.set_queries(gi, so)
+        say("autotune")
+        rep = b.autotune(repeats=1)
+        assert rep["candidates"] > 0 and rep["block_size"] in (512, 1024)
+        buckets = {e["bucket"] for e in rep["buckets"]}
+        assert 18 in buckets and rep["buckets"][-1]["plan"]["tier"] == dfq.TIER_GRID
+        b.run()
+        check_all(b, gi, so, expect)
+
+
+def test_batch_requery_and_errors(dfq, batch_setup):
+    gs, gi, so, expect = batch_setup
+    with dfq.Batch(gs[:3]) as b:
+        with pytest.raises(dfq.DfqError):
+            b.run()  # no queries yet
+        with pytest.raises(dfq.DfqError) as e:
+            b.set_queries([0], [gs[0].n])
+        assert e.value.code == dfq.ERR_RANGE
+        b.set_queries([2, 0], [5, 1])
+        b.run()
+        for q, (g, s) in enumerate(((2, 5), (0, 1))):
+            with dfq.DeviceGraph(gs[g]) as dg:
+                np.testing.assert_array_equal(b.levels(q), dg.bfs(s).level)

@@ -358,11 +358,14 @@ __device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
   __shared__ SharedBatch<BLOCK> sh;
-  BarrierState bst{0, 0, 0, 0, 0};
-  if (!kernel_prologue<BLOCK>(b.grid.ctl, sh.grid, bst)) return;
+  Team team{};
+  GridBarrier *bar = &b.grid.ctl->bar;
+  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh.grid, team)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.grid.ctl->err = 2;
+    return;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) *b.work = 0;
-  grid_barrier(&b.grid.ctl->bar, bst, 0, sh.grid.bcast);
-  if (blockIdx.x == 0 && threadIdx.x == 0) barrier_census_clear(&b.grid.ctl->bar);
+  grid_barrier(bar, team.bst, 0, sh.grid.bcast);
 
   // grid tier: one instance at a time over the whole grid
   for (uint32_t qi = 0; qi < b.n_grid; ++qi) {
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     a.bottom_up = (q.direction_opt && gr.symmetric) ? 1u : 0u;
     a.alpha = q.alpha;
     a.bu_exit = q.beta ? gr.n / q.beta : 0;
-    grid_bfs<BLOCK>(a, sh.grid, bst);
+    grid_bfs<BLOCK>(a, sh.grid, team);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       b.results[qi].depth = b.grid.ctl->depth;
       b.results[qi].reached = b.grid.ctl->reached;

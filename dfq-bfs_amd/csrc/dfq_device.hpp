@@ -101,15 +101,35 @@ struct GridBarrier {
   BarrierWord side_out;         // side payload total of the last round
   BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
   BarrierWord census_total;     // workgroups counted at kernel start
+  BarrierWord census_done;      // workgroups that have read the census
+  BarrierWord ticket;           // workgroup ranks handed out within a team
   uint32_t census[kMaxXcd];     // workgroups per XCD
 };
 
+// Barrier state of one workgroup within its team (thread 0's copy counts).
 struct BarrierState {
   uint32_t gen;     // generation modulo 2^24
   uint32_t xcc;     // this workgroup's XCD
-  uint32_t xcc_n;   // workgroups on this XCD
-  uint32_t nx;      // XCDs with workgroups
-  uint32_t rank;    // index of this XCD among those with workgroups
+  uint32_t xcc_n;   // workgroups of the team on this XCD
+  uint32_t nx;      // XCDs in the team
+};
+
+// What the kernel-start census tells every workgroup.
+struct CensusInfo {
+  uint32_t xcc;                // this workgroup's XCD
+  uint32_t xrank;              // its rank among the XCDs that have workgroups
+  uint32_t nx;                 // XCDs with workgroups
+  uint32_t count[kMaxXcd];     // workgroups per XCD rank
+};
+
+// A team: the workgroups of `xpt` consecutive XCD ranks, with its own
+// barrier words.  The whole grid is one team of all XCDs.
+struct Team {
+  uint32_t id;      // team index
+  uint32_t rank;    // workgroup index within the team
+  uint32_t size;    // workgroups in the team
+  uint32_t seg;     // output segment: this workgroup's XCD rank within the team
+  BarrierState bst; // thread 0 only
 };
 
 __device__ __forceinline__ uint32_t xcc_id() {
@@ -132,33 +152,61 @@ __device__ __forceinline__ T ld_relaxed(T *p) {
 }
 
 // Kernel start (thread 0 of every workgroup, all workgroups resident):
-// counts workgroups per XCD.  The counters are cleared again by
-// barrier_census_clear after the last barrier of the launch.
-__device__ __forceinline__ void barrier_census(GridBarrier *b, BarrierState &st) {
-  st.xcc = xcc_id();
-  st.gen = (uint32_t)(ld_relaxed(&b->gensum.word) >> 40);
+// counts workgroups per XCD.  The last workgroup to finish reading clears
+// the counters for the next launch.
+__device__ __forceinline__ void barrier_census(GridBarrier *b, CensusInfo &ci) {
+  ci.xcc = xcc_id();
   // the per-XCD count must land before the total that releases the readers
-  const uint32_t mine = __hip_atomic_fetch_add(&b->census[st.xcc], 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(&b->census[ci.xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   wait_own_stores();
-  __hip_atomic_fetch_add(&b->census_total.word, (uint64_t)(mine != 0xffffffffu), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(&b->census_total.word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   while (ld_relaxed(&b->census_total.word) < gridDim.x) __builtin_amdgcn_s_sleep(1);
-  st.xcc_n = ld_relaxed(&b->census[st.xcc]);
-  uint32_t nx = 0, rank = 0;
+  uint32_t nx = 0, xrank = 0;
+  for (int i = 0; i < kMaxXcd; ++i) ci.count[i] = 0;
   for (int i = 0; i < kMaxXcd; ++i) {
-    const bool present = ld_relaxed(&b->census[i]) != 0;
-    nx += present;
-    rank += present && (uint32_t)i < st.xcc;
+    const uint32_t c = ld_relaxed(&b->census[i]);
+    if (c == 0) continue;
+    if ((uint32_t)i < ci.xcc) ++xrank;
+    ci.count[nx++] = c;
   }
-  st.nx = nx;
-  st.rank = rank;
+  ci.nx = nx;
+  ci.xrank = xrank;
+  wait_own_stores();
+  const uint64_t done = __hip_atomic_fetch_add(&b->census_done.word, 1ull, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+  if (done + 1 == gridDim.x) {
+    // everyone has read: reset for the next launch
+    for (int i = 0; i < kMaxXcd; ++i)
+      __hip_atomic_store(&b->census[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&b->census_total.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&b->census_done.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
-__device__ __forceinline__ void barrier_census_clear(GridBarrier *b) {
-  for (int i = 0; i < kMaxXcd; ++i)
-    __hip_atomic_store(&b->census[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&b->census_total.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Team of XCD ranks [id * xpt, id * xpt + xpt) (thread 0).  `tb` is the
+// team's barrier.  by_ticket: ranks from the team's ticket counter (reset
+// by team_ticket_reset after the team's first barrier); otherwise the
+// grid is one team and the rank is blockIdx.x.
+__device__ __forceinline__ void team_form(const CensusInfo &ci, uint32_t xpt, GridBarrier *tb, bool by_ticket,
+                                          Team &t) {
+  t.id = ci.xrank / xpt;
+  const uint32_t first = t.id * xpt;
+  const uint32_t last = min(first + xpt, ci.nx);
+  uint32_t size = 0;
+  for (uint32_t r = first; r < last; ++r) size += ci.count[r];
+  t.size = size;
+  t.seg = ci.xrank - first;
+  t.rank = by_ticket ? (uint32_t)__hip_atomic_fetch_add(&tb->ticket.word, 1ull, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                     : blockIdx.x;
+  t.bst.gen = (uint32_t)(ld_relaxed(&tb->gensum.word) >> 40);
+  t.bst.xcc = ci.xcc;
+  t.bst.xcc_n = ci.count[ci.xrank];
+  t.bst.nx = last - first;
+}
+
+__device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
+  __hip_atomic_store(&tb->ticket.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // All threads of every workgroup must call it.  `payload` (< 2^40) and

@@ -139,7 +139,8 @@ struct SharedAll {
   uint32_t stage_base[KernelShape<BLOCK>::kWaves];
   uint32_t seg_pre[kSegments + 1];
   uint32_t item_pre[kSegments + 1];
-  uint32_t seg;  // output segment of this workgroup (its XCD's index)
+  uint32_t seg;  // output segment of this workgroup (its XCD's rank in the team)
+  uint32_t team_rank, team_size, team_id;
   uint32_t wave_sum[KernelShape<BLOCK>::kWaves];
   uint64_t prof[kProfFields];
   uint64_t bcast[2];
@@ -305,7 +306,7 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 template <int BLOCK>
 __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
-                                uint32_t wave, uint64_t &edges) {
+                                uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) {
@@ -324,12 +325,12 @@ __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
   // ones in every workgroup
   const uint32_t total = sh.seg_pre[kSegments];
   const uint32_t nblk = (total + kWave - 1) / kWave;
-  const uint32_t G = gridDim.x, W = (uint32_t)S::kWaves;
-  const uint32_t mine = nblk > blockIdx.x ? (nblk - blockIdx.x - 1) / G + 1 : 0;
+  const uint32_t G = tsize, W = (uint32_t)S::kWaves;
+  const uint32_t mine = nblk > trank ? (nblk - trank - 1) / G + 1 : 0;
   {
     for (uint32_t c0 = 0; c0 < mine; c0 += W) {
       const uint32_t c = c0 + wave;  // this wave's block ordinal within the workgroup
-      const uint32_t p = (blockIdx.x + c * G) * kWave + lane_id();
+      const uint32_t p = (trank + c * G) * kWave + lane_id();
       uint32_t v = 0, deg = 0;
       int64_t start = 0;
       if (c < mine && p < total) {
@@ -504,14 +505,16 @@ __device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, 
 // arguments): initialisation, then the level loop until a level claims no
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
 template <int BLOCK>
-__device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, BarrierState &bst) {
+__device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
-  const uint32_t gtid = blockIdx.x * BLOCK + tid;
-  const uint32_t gstride = gridDim.x * BLOCK;
-  const uint32_t gwave = blockIdx.x * S::kWaves + wave;
-  const uint32_t nwaves = gridDim.x * S::kWaves;
+  const uint32_t trank = sh.team_rank, tsize = sh.team_size;
+  const uint32_t gtid = trank * BLOCK + tid;
+  const uint32_t gstride = tsize * BLOCK;
+  const uint32_t gwave = trank * S::kWaves + wave;
+  const uint32_t nwaves = tsize * S::kWaves;
+  BarrierState &bst = team.bst;
   Control *ctl = a.ctl;
   if (tid < kProfFields) sh.prof[tid] = 0;
 
@@ -589,7 +592,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, BarrierState &b
       // refresh the snapshot: words copied while claims go on are still
       // subsets of the visited set, which is all the filter needs
       for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
-      topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, edges);
+      topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, trank, tsize, edges);
       flush_block<BLOCK>(vis, sh, wave);
       if (a.prof && tid == 0) busy += now_ns() - t_start;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
@@ -641,7 +644,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, BarrierState &b
         sh.prof[kProfBusyMaxNs] = busy;
         sh.prof[kProfBusySumNs] = busy;
       }
-      if (tid == 0 && blockIdx.x == 0) {
+      if (tid == 0 && trank == 0) {
         sh.prof[kProfFrontier] = n_cur;
         sh.prof[kProfClaims] = n_next;
         sh.prof[kProfHuge] = hc;
@@ -666,34 +669,37 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, BarrierState &b
   }
 }
 
-// Census of workgroups per XCD; false (in every workgroup alike) when the
-// device has more XCDs than frontier segments.
-template <int BLOCK>
-__device__ bool kernel_prologue(Control *ctl, SharedAll<BLOCK> &sh, BarrierState &bst) {
+// Kernel start: census of workgroups per XCD and formation of teams of
+// `xpt` XCDs each, whose barriers are team_bar(id).  Returns false in every
+// workgroup alike when a team would span more XCDs than frontier segments.
+template <int BLOCK, class TeamBar>
+__device__ bool kernel_prologue(GridBarrier *census_bar, uint32_t xpt, bool by_ticket, TeamBar team_bar,
+                                SharedAll<BLOCK> &sh, Team &team) {
   if (threadIdx.x == 0) {
-    barrier_census(&ctl->bar, bst);
-    sh.seg = bst.rank;
-    sh.bcast[0] = bst.nx;
+    CensusInfo ci;
+    barrier_census(census_bar, ci);
+    if (xpt == 0 || xpt > ci.nx) xpt = ci.nx;
+    team_form(ci, xpt, team_bar(ci.xrank / xpt), by_ticket, team);
+    sh.seg = team.seg;
+    sh.team_rank = team.rank;
+    sh.team_size = team.size;
+    sh.team_id = team.id;
+    sh.bcast[0] = xpt;
   }
   __syncthreads();
-  if (sh.bcast[0] > (uint64_t)kSegments) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      ctl->err = 2;
-      barrier_census_clear(&ctl->bar);
-    }
-    return false;
-  }
-  return true;
+  return sh.bcast[0] <= (uint64_t)kSegments;
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
-  BarrierState bst{0, 0, 0, 0, 0};
-  if (!kernel_prologue<BLOCK>(a.ctl, sh, bst)) return;
-  grid_bfs<BLOCK>(a, sh, bst);
-  // every workgroup read the census before the search's first barrier
-  if (blockIdx.x == 0 && threadIdx.x == 0) barrier_census_clear(&a.ctl->bar);
+  Team team{};
+  GridBarrier *bar = &a.ctl->bar;
+  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->err = 2;
+    return;
+  }
+  grid_bfs<BLOCK>(a, sh, team);
 }
 
 }  // namespace dfq

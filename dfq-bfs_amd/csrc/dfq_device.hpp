@@ -129,6 +129,7 @@ struct Team {
   uint32_t rank;    // workgroup index within the team
   uint32_t size;    // workgroups in the team
   uint32_t seg;     // output segment: this workgroup's XCD rank within the team
+  uint32_t teams;   // teams in the grid
   BarrierState bst; // thread 0 only
 };
 
@@ -141,6 +142,11 @@ __device__ __forceinline__ uint32_t xcc_id() {
 __device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void l2_writeback() {
   asm volatile("buffer_wbl2 sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+}
+// Write-back at system scope: the lines become visible to the host and to
+// copy engines, not only to the other XCDs.
+__device__ __forceinline__ void l2_writeback_system() {
+  asm volatile("buffer_wbl2 sc0 sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
 }
 __device__ __forceinline__ void l2_invalidate() {
   asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
@@ -196,6 +202,7 @@ __device__ __forceinline__ void team_form(const CensusInfo &ci, uint32_t xpt, Gr
   for (uint32_t r = first; r < last; ++r) size += ci.count[r];
   t.size = size;
   t.seg = ci.xrank - first;
+  t.teams = (ci.nx + xpt - 1) / xpt;
   t.rank = by_ticket ? (uint32_t)__hip_atomic_fetch_add(&tb->ticket.word, 1ull, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT)
                      : blockIdx.x;
@@ -213,6 +220,9 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
 // `side` are read from thread 0; returns the grid-wide payload sum in every
 // thread and, when side_total is given, the side sum there.  `st` is only
 // meaningful in thread 0.  `s_word` points at two 64-bit LDS slots.
+// kSystem: the XCD leaders write their L2 back at system scope, so every
+// store made before the barrier can be read by the host or a DMA copy.
+template <bool kSystem = false>
 __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &st, uint64_t payload,
                                                  uint64_t *s_word, uint64_t side = 0,
                                                  uint64_t *side_total = nullptr) {
@@ -235,7 +245,9 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
       __hip_atomic_store(&b->xleaf[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&b->xside[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (xs) __hip_atomic_fetch_add(&b->root_side.word, xs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      l2_writeback();  // also waits for the stores and atomics above
+      // also waits for the stores and atomics above
+      if (kSystem) l2_writeback_system();
+      else l2_writeback();
       const uint64_t o2 = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
       if ((o2 >> 40) + 1 == st.nx) {

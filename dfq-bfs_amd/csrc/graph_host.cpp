@@ -224,6 +224,8 @@ const char *dfq_strerror(int status) {
     case DFQ_ERR_CAPACITY: return "capacity exceeded";
     case DFQ_ERR_BUSY: return "busy: admission refused";
     case DFQ_ERR_STATE: return "invalid state";
+    case DFQ_ERR_PENDING: return "query still running";
+    case DFQ_ERR_TIMEOUT: return "timed out";
     default: return "unknown error";
   }
 }

@@ -75,6 +75,29 @@ void parallel_for(uint64_t n, F fn, unsigned threads = 0) {
   for (auto &th : pool) th.join();
 }
 
+// CSR input check shared by the device front ends: offsets start at 0, end
+// at m and never decrease; every column index is a vertex.
+inline bool csr_valid(const int64_t *off, const uint32_t *col, uint32_t n, uint64_t m) {
+  if (off[0] != 0 || (uint64_t)off[n] != m) return false;
+  std::atomic<bool> bad{false};
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t v = b; v < e; ++v)
+      if (off[v + 1] < off[v]) {
+        bad = true;
+        return;
+      }
+  });
+  if (bad) return false;
+  parallel_for(m, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t i = b; i < e; ++i)
+      if (col[i] >= n) {
+        bad = true;
+        return;
+      }
+  });
+  return !bad;
+}
+
 // Bottom-up levels scan a vertex's out-edges as its in-edges, which is only
 // valid when every u->v has a v->u.  Requires sorted rows (binary search);
 // a graph with unsorted rows is treated as directed.

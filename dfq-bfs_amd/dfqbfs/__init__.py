@@ -1,4 +1,5 @@
-"""ctypes view of the MI355X DFQ-BFS C ABI (include/dfq_graph.h, include/dfq_bfs.h).
+"""ctypes view of the MI355X DFQ-BFS C ABI (include/dfq_graph.h, dfq_bfs.h,
+dfq_batch.h, dfq_serve.h).
 
 The product is the native library ``lib/libdfqbfs.so`` (C++ host code and
 HIP kernels).  This module only binds its entry points for the tests and the
@@ -39,6 +40,8 @@ ERR_COOPERATIVE = -8
 ERR_CAPACITY = -9
 ERR_BUSY = -10
 ERR_STATE = -11
+ERR_PENDING = -12
+ERR_TIMEOUT = -13
 
 CSR_SYMMETRIZE = 0x1
 CSR_DEDUP = 0x2
@@ -549,3 +552,182 @@ class Batch:
                 "buckets": [dict(bucket=e.bucket, queries=e.queries, plan=e.plan.as_dict(),
                                  cost_us=e.cost_us, sub_batch_ms=e.sub_batch_ms)
                             for e in rep.entry[:rep.entries]]}
+
+
+# ---------------------------------------------------------------------------
+# Serving (include/dfq_serve.h)
+# ---------------------------------------------------------------------------
+SERVE_PARENTS = 0x1
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("xcds_per_team", C.c_uint32), ("ring_slots", C.c_uint32),
+                ("reserved", C.c_uint32), ("arena_bytes", C.c_uint64)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("running", C.c_uint32), ("teams", C.c_uint32), ("grid_blocks", C.c_uint32),
+                ("block_size", C.c_uint32), ("xcds_per_team", C.c_uint32), ("ring_slots", C.c_uint32),
+                ("in_flight", C.c_uint32), ("reserved", C.c_uint32), ("arena_bytes", C.c_uint64),
+                ("arena_used", C.c_uint64), ("team_blocks", C.c_uint32 * 16)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "team_blocks")}
+        d["team_blocks"] = list(self.team_blocks[:self.teams])
+        return d
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("graph", C.c_uint32), ("source", C.c_uint32), ("depth", C.c_uint32),
+                ("bottom_up_levels", C.c_uint32), ("reached", C.c_uint64), ("team", C.c_uint32),
+                ("reserved", C.c_uint32), ("service_us", C.c_double)]
+
+
+class ServeRecord(C.Structure):
+    _fields_ = [("arrival_us", C.c_double), ("admit_us", C.c_double), ("done_us", C.c_double),
+                ("service_us", C.c_double), ("reached", C.c_uint64), ("depth", C.c_uint32),
+                ("team", C.c_uint32)]
+
+
+_sig("dfq_serve_config_default", C.c_int, C.POINTER(ServeConfig))
+_sig("dfq_serve_footprint", C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32)
+_sig("dfq_server_create", C.c_int, C.c_uint32, _u32p, C.POINTER(C.c_uint64), _i64p, _u32p,
+     C.POINTER(ServeConfig), C.POINTER(_P))
+_sig("dfq_server_destroy", None, _P)
+_sig("dfq_server_set_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
+_sig("dfq_server_get_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
+_sig("dfq_server_set_shape", C.c_int, _P, C.c_uint32, C.c_uint32)
+_sig("dfq_server_get_info", C.c_int, _P, C.POINTER(ServerInfo))
+_sig("dfq_server_start", C.c_int, _P)
+_sig("dfq_server_stop", C.c_int, _P)
+_sig("dfq_server_submit", C.c_int, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64))
+_sig("dfq_server_poll", C.c_int, _P, C.c_uint64, C.POINTER(ServeResult))
+_sig("dfq_server_wait", C.c_int, _P, C.c_uint64, C.c_uint32, C.POINTER(ServeResult))
+_sig("dfq_server_levels", C.c_int, _P, C.c_uint64, _u32p, _u32p)
+_sig("dfq_server_release", C.c_int, _P, C.c_uint64)
+_sig("dfq_server_replay", C.c_int, _P, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_double), C.c_uint32,
+     C.POINTER(_u32p), C.POINTER(ServeRecord))
+
+
+def default_serve_config() -> ServeConfig:
+    c = ServeConfig()
+    check(lib.dfq_serve_config_default(C.byref(c)), "dfq_serve_config_default")
+    return c
+
+
+def serve_footprint(n: int, m: int, large_max: int = 256, xcds_per_team: int = 1, flags: int = 0) -> int:
+    return int(lib.dfq_serve_footprint(n, m, large_max, xcds_per_team, flags))
+
+
+class Server:
+    """Resident graphs served by one persistent launch (continuous batching)."""
+
+    def __init__(self, graphs, *, xcds_per_team: int | None = None, block_size: int | None = None,
+                 ring_slots: int | None = None, arena_bytes: int | None = None):
+        self.graphs = list(graphs)
+        cfg = default_serve_config()
+        if xcds_per_team is not None:
+            cfg.xcds_per_team = xcds_per_team
+        if block_size is not None:
+            cfg.block_size = block_size
+        if ring_slots is not None:
+            cfg.ring_slots = ring_slots
+        if arena_bytes is not None:
+            cfg.arena_bytes = arena_bytes
+        n = np.array([g.n for g in self.graphs], np.uint32)
+        m = np.array([g.m for g in self.graphs], np.uint64)
+        ro = np.concatenate([g.row_offsets for g in self.graphs]).astype(np.int64)
+        ci = (np.concatenate([g.col_idx for g in self.graphs]).astype(np.uint32)
+              if m.sum() else np.zeros(1, np.uint32))
+        self.n = n
+        h = _P()
+        check(lib.dfq_server_create(len(self.graphs), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                    _i64(ro), _u32(ci), C.byref(cfg), C.byref(h)), "dfq_server_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.dfq_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        i = ServerInfo()
+        check(lib.dfq_server_get_info(self._h, C.byref(i)), "dfq_server_get_info")
+        return i.as_dict()
+
+    def plan(self, graph: int) -> QueryPlan:
+        p = QueryPlan()
+        check(lib.dfq_server_get_plan(self._h, graph, C.byref(p)), "dfq_server_get_plan")
+        return p
+
+    def set_plan(self, graph: int, **kw):
+        p = self.plan(graph)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        check(lib.dfq_server_set_plan(self._h, graph, C.byref(p)), "dfq_server_set_plan")
+
+    def set_shape(self, xcds_per_team: int, block_size: int = 1024):
+        check(lib.dfq_server_set_shape(self._h, xcds_per_team, block_size), "dfq_server_set_shape")
+
+    def start(self):
+        check(lib.dfq_server_start(self._h), "dfq_server_start")
+
+    def stop(self):
+        check(lib.dfq_server_stop(self._h), "dfq_server_stop")
+
+    def submit(self, graph: int, source: int, flags: int = 0) -> int:
+        """Ticket of the admitted query; raises DfqError (code ERR_BUSY) when refused for now."""
+        t = C.c_uint64()
+        check(lib.dfq_server_submit(self._h, graph, source, flags, C.byref(t)), "dfq_server_submit")
+        return t.value
+
+    def poll(self, ticket: int):
+        r = ServeResult()
+        rc = lib.dfq_server_poll(self._h, ticket, C.byref(r))
+        if rc == ERR_PENDING:
+            return None
+        check(rc, "dfq_server_poll")
+        return r
+
+    def wait(self, ticket: int, timeout_ms: int = 60000) -> ServeResult:
+        r = ServeResult()
+        check(lib.dfq_server_wait(self._h, ticket, timeout_ms, C.byref(r)), "dfq_server_wait")
+        return r
+
+    def levels(self, ticket: int, graph: int, parents: bool = False):
+        lv = np.empty(int(self.n[graph]), np.uint32)
+        pa = np.empty(int(self.n[graph]), np.uint32) if parents else None
+        check(lib.dfq_server_levels(self._h, ticket, _u32(lv), _u32(pa) if parents else None),
+              "dfq_server_levels")
+        return (lv, pa) if parents else lv
+
+    def release(self, ticket: int):
+        check(lib.dfq_server_release(self._h, ticket), "dfq_server_release")
+
+    def replay(self, graph_ids, sources, arrival_us, flags: int = 0, keep_levels: bool = False):
+        """Serves a trace; returns (records as a structured array, levels or None)."""
+        gi = np.ascontiguousarray(graph_ids, np.uint32)
+        so = np.ascontiguousarray(sources, np.uint32)
+        ar = np.ascontiguousarray(arrival_us, np.float64)
+        nq = gi.size
+        recs = (ServeRecord * max(nq, 1))()
+        outs = None
+        out_ptrs = None
+        if keep_levels:
+            outs = [np.empty(int(self.n[g]), np.uint32) for g in gi]
+            out_ptrs = (_u32p * max(nq, 1))(*[_u32(o) for o in outs])
+        check(lib.dfq_server_replay(self._h, nq, _u32(gi), _u32(so), ar.ctypes.data_as(C.POINTER(C.c_double)),
+                                    flags, out_ptrs, recs), "dfq_server_replay")
+        dt = np.dtype([("arrival_us", "f8"), ("admit_us", "f8"), ("done_us", "f8"), ("service_us", "f8"),
+                       ("reached", "u8"), ("depth", "u4"), ("team", "u4")])
+        arr = np.frombuffer(bytearray(recs), dtype=dt, count=nq).copy() if nq else np.zeros(0, dt)
+        return arr, outs

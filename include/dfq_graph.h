@@ -44,6 +44,8 @@ extern "C" {
 #define DFQ_ERR_CAPACITY      -9  /* a queue / slot / memory budget exceeded    */
 #define DFQ_ERR_BUSY         -10  /* serving loop: admission refused for now    */
 #define DFQ_ERR_STATE        -11  /* call not valid in the object's state       */
+#define DFQ_ERR_PENDING      -12  /* serving loop: query not finished yet       */
+#define DFQ_ERR_TIMEOUT      -13  /* a bounded wait ran out                     */
 
 /* Unreached vertex marker in level and parent arrays. */
 #define DFQ_UNREACHED 0xffffffffu

@@ -37,7 +37,8 @@ def test_every_declared_symbol_is_exported(dfq):
 
 def test_headers_compile_as_c(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "dfq_bfs.h"\n#include "dfq_graph.h"\nint main(void){return 0;}\n')
+    src.write_text('#include "dfq_bfs.h"\n#include "dfq_graph.h"\n#include "dfq_batch.h"\n'
+                   '#include "dfq_serve.h"\nint main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                     "-c", str(src), "-o", str(tmp_path / "t.o")], check=True)
 
@@ -69,3 +70,55 @@ def test_graph_create_validates_input_before_device(dfq):
     with pytest.raises(dfq.DfqError) as e:
         dfq.DeviceGraph(row_offsets=np.array([0, 1, 2]), col_idx=np.array([1, 2], np.uint32))
     assert e.value.code == dfq.ERR_RANGE
+
+
+def test_serve_config_and_footprint(dfq):
+    """Serving defaults and the admission footprint (no GPU needed)."""
+    c = dfq.default_serve_config()
+    assert c.block_size == 1024 and c.xcds_per_team == 1 and c.ring_slots == 1024 and c.arena_bytes == 0
+    n, m = 1 << 20, 16 << 20
+
+    def al(x):
+        return (x + 255) // 256 * 256
+    words = (n + 31) // 32
+    # item capacity: m / 256 + min(n, m / (large_max + 1)) + 1
+    cap = m // 256 + min(n, m // 257) + 1
+    for xpt, segs in ((1, 1), (2, 2), (8, 8), (0, 8)):
+        expect = (al(4 * n) + 2 * al(4 * words) + al(4 * (words + 1)) + 2 * al(4 * segs * n)
+                  + al(8 * segs * cap))
+        assert dfq.serve_footprint(n, m, 256, xpt, 0) == expect
+        assert dfq.serve_footprint(n, m, 256, xpt, dfq.SERVE_PARENTS) == expect + al(4 * n)
+    # a larger row cut-off can only shrink the work-item bound
+    assert dfq.serve_footprint(n, m, 4096, 1, 0) < dfq.serve_footprint(n, m, 256, 1, 0)
+    # tiny graph: no CSR entries still has room for one item per segment
+    assert dfq.serve_footprint(1, 0, 256, 1, 0) == al(4) * 3 + al(8) + 2 * al(4) + al(8)
+
+
+def test_server_create_rejects_bad_input(dfq):
+    ro = np.array([0, 1, 2], np.int64)
+    ci = np.array([1, 0], np.uint32)
+    n = np.array([2], np.uint32)
+    m = np.array([2], np.uint64)
+    h = dfq._P()
+    mp = m.ctypes.data_as(dfq.C.POINTER(dfq.C.c_uint64))
+    cfg = dfq.default_serve_config()
+    cfg.ring_slots = 1000  # not a power of two
+    assert dfq.lib.dfq_server_create(1, dfq._u32(n), mp, dfq._i64(ro), dfq._u32(ci), dfq.C.byref(cfg),
+                                     dfq.C.byref(h)) == dfq.ERR_INVALID_ARG
+    cfg = dfq.default_serve_config()
+    cfg.xcds_per_team = 3
+    assert dfq.lib.dfq_server_create(1, dfq._u32(n), mp, dfq._i64(ro), dfq._u32(ci), dfq.C.byref(cfg),
+                                     dfq.C.byref(h)) == dfq.ERR_INVALID_ARG
+    bad = np.array([0, 1, 3], np.int64)  # last offset != m
+    assert dfq.lib.dfq_server_create(1, dfq._u32(n), mp, dfq._i64(bad), dfq._u32(ci), None,
+                                     dfq.C.byref(h)) == dfq.ERR_RANGE
+    # the rest needs a device; without one the create call reports it
+    if not gpu_available():
+        assert dfq.lib.dfq_server_create(1, dfq._u32(n), mp, dfq._i64(ro), dfq._u32(ci), None,
+                                         dfq.C.byref(h)) == dfq.ERR_NO_DEVICE
+    # calls on a null server are argument errors, not crashes
+    t = dfq.C.c_uint64()
+    assert dfq.lib.dfq_server_submit(None, 0, 0, 0, dfq.C.byref(t)) == dfq.ERR_INVALID_ARG
+    assert dfq.lib.dfq_server_start(None) == dfq.ERR_INVALID_ARG
+    assert dfq.lib.dfq_server_release(None, 0) == dfq.ERR_INVALID_ARG
+    assert dfq.lib.dfq_strerror(dfq.ERR_PENDING) == b"query still running"

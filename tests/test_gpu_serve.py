@@ -1,0 +1,180 @@
+"""Served searches (include/dfq_serve.h): one persistent launch, continuous
+batching.  Every query's levels equal the oracle's, and a query gives the
+same levels alone as in the mix of a trace where many run at once."""
+import numpy as np
+import pytest
+
+from conftest import UNREACHED
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_graphs(dfq):
+    return [
+        dfq.Csr.rmat(12, 16, seed=212),           # symmetric, bottom-up levels
+        dfq.Csr.rmat(15, 16, seed=215),
+        dfq.Csr.rmat(17, 8, seed=317),
+        dfq.Csr.amazon_like(20000, seed=7),       # directed: top-down only
+        dfq.Csr.grid2d(200, 150, 0.03, seed=5),   # several hundred levels
+    ]
+
+
+@pytest.fixture(scope="module")
+def serve_setup(dfq, oracle):
+    gs = make_graphs(dfq)
+    rng = np.random.default_rng(55)
+    nq = 160
+    gi = rng.integers(0, len(gs), nq).astype(np.uint32)
+    so = np.array([int(gs[g].pick_sources(1, seed=1000 + q)[0]) for q, g in enumerate(gi)], np.uint32)
+    # Poisson arrivals, 20 us apart on average: far more than one query in flight
+    arrival = np.cumsum(rng.exponential(20.0, nq))
+    expect = {}
+    for g, s in zip(gi, so):
+        if (g, s) not in expect:
+            expect[(g, s)] = oracle.bfs(gs[g].row_offsets, gs[g].col_idx, int(s))
+    return gs, gi, so, arrival, expect
+
+
+def check_record(rec, e):
+    assert rec["reached"] == int((e != UNREACHED).sum())
+    assert rec["depth"] == int(e[e != UNREACHED].max()) + 1
+
+
+def max_overlap(recs):
+    ev = [(r["admit_us"], 1) for r in recs] + [(r["done_us"], -1) for r in recs]
+    cur = best = 0
+    for _, d in sorted(ev, key=lambda x: (x[0], x[1])):
+        cur += d
+        best = max(best, cur)
+    return best
+
+
+def test_serve_mix_equals_alone_and_oracle(dfq, serve_setup):
+    gs, gi, so, arrival, expect = serve_setup
+    with dfq.Server(gs, xcds_per_team=1) as srv:
+        srv.start()
+        info = srv.info()
+        say("server", info)
+        assert info["running"] == 1 and info["teams"] >= 1
+        assert sum(info["team_blocks"]) == info["grid_blocks"]
+        say("replay", len(gi), "queries")
+        recs, outs = srv.replay(gi, so, arrival, keep_levels=True)
+        for q in range(len(gi)):
+            e = expect[(gi[q], so[q])]
+            np.testing.assert_array_equal(outs[q], e, err_msg=f"mix: query {q} graph {gi[q]} source {so[q]}")
+            check_record(recs[q], e)
+        assert (recs["done_us"] >= recs["admit_us"]).all() and (recs["admit_us"] >= recs["arrival_us"]).all()
+        say("max in flight", max_overlap(recs), "teams used", sorted(set(recs["team"].tolist())))
+        if info["teams"] > 1:
+            assert max_overlap(recs) > 1
+            assert len(set(recs["team"].tolist())) > 1
+        # the same queries one at a time
+        for q in range(0, len(gi), 4):
+            t = srv.submit(int(gi[q]), int(so[q]))
+            r = srv.wait(t)
+            lv = srv.levels(t, int(gi[q]))
+            srv.release(t)
+            np.testing.assert_array_equal(lv, outs[q], err_msg=f"alone: query {q}")
+            assert r.reached == recs[q]["reached"] and r.depth == recs[q]["depth"]
+        assert srv.info()["in_flight"] == 0
+        srv.stop()
+        assert srv.info()["running"] == 0
+
+
+@pytest.mark.parametrize("xpt,block", [(2, 1024), (4, 1024), (0, 1024), (1, 512)])
+def test_serve_team_shapes(dfq, serve_setup, xpt, block):
+    gs, gi, so, arrival, expect = serve_setup
+    sel = slice(0, 60)
+    with dfq.Server(gs, xcds_per_team=xpt, block_size=block) as srv:
+        srv.start()
+        say("shape", xpt, block, srv.info())
+        recs, outs = srv.replay(gi[sel], so[sel], arrival[sel], keep_levels=True)
+        for q in range(len(recs)):
+            e = expect[(gi[q], so[q])]
+            np.testing.assert_array_equal(outs[q], e, err_msg=f"xpt {xpt}: query {q} graph {gi[q]}")
+            check_record(recs[q], e)
+        srv.stop()
+
+
+def test_serve_parents_and_direction_plans(dfq, serve_setup):
+    gs, gi, so, arrival, expect = serve_setup
+    with dfq.Server(gs) as srv:
+        srv.set_plan(1, direction_opt=0, small_max=16, medium_max=64, large_max=1024)
+        srv.start()
+        tickets = []
+        for q in range(0, 40, 2):
+            tickets.append((q, srv.submit(int(gi[q]), int(so[q]), dfq.SERVE_PARENTS)))
+        for q, t in tickets:
+            srv.wait(t)
+            lv, pa = srv.levels(t, int(gi[q]), parents=True)
+            srv.release(t)
+            np.testing.assert_array_equal(lv, expect[(gi[q], so[q])], err_msg=f"query {q}")
+            rep = gs[gi[q]].validate(int(so[q]), lv, pa)
+            assert rep["errors"] == 0, (q, rep)
+        srv.stop()
+
+
+def test_serve_admission_by_footprint(dfq, oracle, serve_setup):
+    gs, gi, so, arrival, expect = serve_setup
+    fp = [dfq.serve_footprint(g.n, g.m, 256, 1, 0) for g in gs]
+    big = int(np.argmax(fp))
+    small = int(np.argmin(fp))
+    # room for one working set of the largest graph only
+    arena = fp[big] + fp[small] // 2
+    with dfq.Server(gs, arena_bytes=arena) as srv:
+        srv.start()
+        info = srv.info()
+        assert info["arena_bytes"] >= arena
+        s_big = int(gs[big].pick_sources(1, seed=1)[0])
+        t1 = srv.submit(big, s_big)
+        with pytest.raises(dfq.DfqError) as ei:
+            srv.submit(big, s_big)        # held range leaves no room until released
+        assert ei.value.code == dfq.ERR_BUSY
+        srv.wait(t1)
+        with pytest.raises(dfq.DfqError) as ei:
+            srv.submit(big, s_big)        # finished but not released: still held
+        assert ei.value.code == dfq.ERR_BUSY
+        np.testing.assert_array_equal(srv.levels(t1, big),
+                                      oracle.bfs(gs[big].row_offsets, gs[big].col_idx, s_big))
+        srv.release(t1)
+        t2 = srv.submit(big, s_big)
+        srv.wait(t2)
+        srv.release(t2)
+        # the trace still completes, one large working set at a time
+        recs, outs = srv.replay(gi[:50], so[:50], np.zeros(50), keep_levels=True)
+        for q in range(50):
+            np.testing.assert_array_equal(outs[q], expect[(gi[q], so[q])], err_msg=f"query {q}")
+        srv.stop()
+    # a working set larger than the whole arena is refused outright
+    with dfq.Server(gs, arena_bytes=fp[small] * 2) as srv:
+        srv.start()
+        with pytest.raises(dfq.DfqError) as ei:
+            srv.submit(big, 0)
+        assert ei.value.code == dfq.ERR_CAPACITY
+        srv.stop()
+
+
+def test_serve_restart_and_results_after_stop(dfq, serve_setup):
+    gs, gi, so, arrival, expect = serve_setup
+    with dfq.Server(gs, ring_slots=8) as srv:
+        for rnd in range(3):
+            srv.start()
+            say("restart round", rnd, srv.info()["teams"])
+            # more queries than the ring holds: slots are reused
+            recs, outs = srv.replay(gi[:20], so[:20], np.zeros(20), keep_levels=True)
+            for q in range(20):
+                np.testing.assert_array_equal(outs[q], expect[(gi[q], so[q])], err_msg=f"round {rnd} q {q}")
+            held = [(q, srv.submit(int(gi[q]), int(so[q]))) for q in range(rnd * 5, rnd * 5 + 5)]
+            for q, t in held:
+                srv.wait(t)
+            srv.stop()
+            # finished queries stay readable after the launch ended
+            for q, t in held:
+                assert srv.poll(t) is not None
+                np.testing.assert_array_equal(srv.levels(t, int(gi[q])), expect[(gi[q], so[q])])
+                srv.release(t)
+            assert srv.info()["in_flight"] == 0

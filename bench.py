@@ -196,6 +196,132 @@ def run_batch(args, dfq, info):
     print(json.dumps(out), flush=True)
 
 
+SERVE_SCALES = (18, 19, 20, 21, 22, 22, 23, 24)
+
+
+def serve_trace(graphs, scales, nq, rate_qps, seed):
+    """Queries of the served mix: graph i with p ~ 1/scale_i, a uniform source
+    with degree > 0 in it; Poisson arrivals at rate_qps (all at 0 if None)."""
+    rng = np.random.default_rng(seed)
+    p = 1.0 / np.array(scales, float)
+    gi = rng.choice(len(graphs), size=nq, p=p / p.sum()).astype(np.uint32)
+    so = np.zeros(nq, np.uint32)
+    for g in range(len(graphs)):
+        idx = np.nonzero(gi == g)[0]
+        if idx.size:
+            so[idx] = graphs[g].pick_sources(int(idx.size), seed=seed * 100 + g)
+    arrival = np.cumsum(rng.exponential(1e6 / rate_qps, nq)) if rate_qps else np.zeros(nq)
+    return gi, so, arrival
+
+
+def run_serve(args, dfq, info):
+    """Served form: resident graphs, one persistent launch of XCD teams,
+    continuous batching under Poisson arrivals, against the same queries
+    issued as back-to-back plain launches."""
+    t0 = time.perf_counter()
+    scales = SERVE_SCALES
+    graphs = [dfq.Csr.rmat(s, 16, 0.57, 0.19, 0.19, seed=5000 + i) for i, s in enumerate(scales)]
+    log(f"{len(graphs)} graphs, {sum(g.n for g in graphs)} vertices, {sum(g.m for g in graphs)} entries, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+
+    # back-to-back plain launches on a sample of the mix
+    pg, ps, _ = serve_trace(graphs, scales, args.serve_plain, None, seed=7)
+    dgs = [dfq.DeviceGraph(g) for g in graphs]
+    for dg in dgs:
+        dg.set_config(direction_opt=1)
+    dbuf = dfq.DeviceBuffer(4 * max(g.n for g in graphs))
+    for i, dg in enumerate(dgs):
+        dg.bfs_device(int(graphs[i].pick_sources(1, seed=1)[0]), dbuf.ptr)
+    dfq.synchronize()
+    plain = np.zeros((len(pg), 2), np.uint64)
+    t0 = time.perf_counter()
+    for q in range(len(pg)):
+        st = dgs[pg[q]].bfs_device(int(ps[q]), dbuf.ptr)
+        plain[q] = (st.reached, st.depth)
+    plain_wall = time.perf_counter() - t0
+    plain_qps = len(pg) / plain_wall
+    k_full = min(24, len(pg))
+    plain_levels = [dgs[pg[q]].bfs(int(ps[q])).level for q in range(k_full)]
+    log(f"plain back-to-back: {len(pg)} queries in {plain_wall:.2f}s = {plain_qps:.1f} q/s")
+    for dg in dgs:
+        dg.close()
+    dbuf.free()
+    del dgs
+
+    srv = dfq.Server(graphs, xcds_per_team=1)
+    # team width: the closed-loop rate of the plain sample at each shape
+    sweep = {}
+    exact = True
+    for xpt in args.serve_shapes:
+        srv.set_shape(xpt)
+        srv.start()
+        recs, _ = srv.replay(pg, ps, np.zeros(len(pg)))
+        sweep[xpt] = len(pg) / (recs["done_us"].max() * 1e-6)
+        exact &= bool(np.array_equal(recs["reached"], plain[:, 0]) and
+                      np.array_equal(recs["depth"].astype(np.uint64), plain[:, 1]))
+        srv.stop()
+        log(f"  xcds/team {xpt}: {sweep[xpt]:.1f} q/s closed loop on the plain sample")
+    best = max(sweep, key=sweep.get)
+    srv.set_shape(best)
+    srv.start()
+    info_s = srv.info()
+    # full level arrays of a few queries, served in the mix, against the plain engine
+    _, outs = srv.replay(pg[:k_full], ps[:k_full], np.zeros(k_full), keep_levels=True)
+    exact &= all(np.array_equal(a, b) for a, b in zip(outs, plain_levels))
+
+    # saturation: every arrival at t = 0, admission paces the mix
+    gi, so, _ = serve_trace(graphs, scales, args.serve_queries, None, seed=11)
+    recs, _ = srv.replay(gi, so, np.zeros(len(gi)))
+    sat_s = recs["done_us"].max() * 1e-6
+    sat_qps = len(gi) / sat_s
+    sat_gteps = float(recs["edges"].sum()) / 2 / sat_s / 1e9
+    log(f"saturation: {len(gi)} queries in {sat_s:.2f}s = {sat_qps:.1f} q/s, {sat_gteps:.1f} GTEPS")
+
+    # Poisson arrivals at a fraction of the saturated rate: latency
+    rate = args.serve_load * sat_qps
+    gi2, so2, arr = serve_trace(graphs, scales, args.serve_queries, rate, seed=13)
+    recs2, _ = srv.replay(gi2, so2, arr)
+    lat = (recs2["done_us"] - recs2["arrival_us"]) / 1e3
+    wait = (recs2["admit_us"] - recs2["arrival_us"]) / 1e3
+    span = (recs2["done_us"].max() - recs2["arrival_us"].min()) * 1e-6
+    srv.stop()
+    srv.close()
+    lat_by_scale = {}
+    for g, sc in enumerate(scales):
+        m = gi2 == g
+        if m.any():
+            lat_by_scale[f"g{g}_s{sc}"] = {"p50_ms": round(float(np.percentile(lat[m], 50)), 3),
+                                           "p99_ms": round(float(np.percentile(lat[m], 99)), 3),
+                                           "service_ms_mean": round(float(recs2["service_us"][m].mean()) / 1e3, 3)}
+    out = {
+        "metric": "served BFS queries/sec vs back-to-back plain launches, with p50/p99 latency",
+        "value": round(sat_qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": 1,
+        "warmup": len(args.serve_shapes), "ms_per_step": round(sat_s * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "8 resident undirected RMAT graphs (scales 18,19,20,21,22,22,23,24, "
+                               "edgefactor 16); graph i drawn with p ~ 1/scale_i, uniform source with "
+                               "degree > 0",
+                   "queries": len(gi), "vertices_total": int(sum(g.n for g in graphs)),
+                   "csr_entries_total": int(sum(g.m for g in graphs)),
+                   "xcds_per_team": best, "teams": info_s["teams"], "block_size": info_s["block_size"],
+                   "arena_bytes": info_s["arena_bytes"]},
+        "plain_queries_per_s": round(plain_qps, 1),
+        "plain_sample": len(pg),
+        "speedup_vs_plain": round(sat_qps / plain_qps, 2),
+        "gteps_served": round(sat_gteps, 1),
+        "poisson": {"rate_qps": round(rate, 1), "load": args.serve_load, "queries": len(gi2),
+                    "achieved_qps": round(len(gi2) / span, 1),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
+                    "admission_wait_p99_ms": round(float(np.percentile(wait, 99)), 3),
+                    "by_graph": lat_by_scale},
+        "shape_sweep_qps": {str(k): round(v, 1) for k, v in sweep.items()},
+        "results_exact_vs_plain": exact,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,10 +337,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy of BYTES (counter calibration)")
-    ap.add_argument("--workload", choices=["plain", "batch"], default="plain",
-                    help="plain: the RMAT-22 headline; batch: batched queries (one launch)")
+    ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain",
+                    help="plain: the RMAT-22 headline; batch: batched queries (one launch); "
+                         "serve: resident graphs under Poisson arrivals")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--batch-sources", type=int, default=4)
+    ap.add_argument("--serve-queries", type=int, default=100000)
+    ap.add_argument("--serve-plain", type=int, default=2000, help="plain back-to-back sample")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="Poisson rate as a fraction of the saturated rate")
+    ap.add_argument("--serve-shapes", type=int, nargs="+", default=[1, 2, 4],
+                    help="XCDs per team to compare (the fastest serves the trace)")
     ap.add_argument("--then-copy", type=int, default=0, metavar="BYTES",
                     help="after the benchmark, run the streaming copy of BYTES once (so one "
                          "counter pass covers the BFS and its calibration)")
@@ -236,6 +369,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args, dfq, info)
+        return
+    if args.workload == "serve":
+        run_serve(args, dfq, info)
         return
     t0 = time.perf_counter()
     csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)

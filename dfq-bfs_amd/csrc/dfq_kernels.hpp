@@ -58,7 +58,9 @@ struct Control {
   uint32_t err;       // nonzero: a queue capacity was exceeded
   uint64_t reached;   // vertices with a level
   uint32_t bu_levels; // levels run bottom-up
-  uint32_t pad[59];
+  uint32_t pad0;
+  uint64_t edges;     // CSR entries of reached vertices (when counted)
+  uint32_t pad[56];
 };
 
 // Profiling counters, one row per level (optional).
@@ -102,6 +104,7 @@ struct BfsArgs {
   uint32_t bottom_up;    // direction optimisation allowed (symmetric graph)
   uint32_t alpha;        // bottom-up when frontier edges > unexplored edges / alpha
   uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer vertices
+  uint32_t count_edges;  // sum the out-degrees of reached vertices into ctl->edges
   uint64_t m;            // CSR entries
 };
 
@@ -554,7 +557,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
   vis.atomics = 0;
 
   vis.off = a.off;
-  vis.track_mf = a.bottom_up != 0;
+  vis.track_mf = a.bottom_up != 0 || a.count_edges != 0;
   vis.mf = 0;
 
   uint64_t n_cur = 1, n_prev = 0, reached = 1;
@@ -661,6 +664,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
         ctl->depth = L + 1;
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
+        ctl->edges = vis.track_mf ? m_seen : 0;
       }
       break;
     }

@@ -207,6 +207,7 @@ bool collect(dfq_server *s, uint32_t idx) {
     q.result.depth = r->depth;
     q.result.bottom_up_levels = r->bu_levels;
     q.result.reached = r->reached;
+    q.result.edges = r->edges;
     q.result.team = r->team;
     q.result.service_us = (double)(r->t_end_ns - r->t_start_ns) / 1000.0;
     q.done = true;
@@ -713,6 +714,7 @@ int dfq_server_replay(dfq_server *s, uint32_t nq, const uint32_t *graph, const u
       rec.done_us = us_since(t0);
       rec.service_us = r.service_us;
       rec.reached = r.reached;
+      rec.edges = r.edges;
       rec.depth = r.depth;
       rec.team = r.team;
       if (level_out && level_out[q]) {

@@ -51,7 +51,8 @@ struct alignas(64) ServeResult {
   uint64_t t_start_ns, t_end_ns;  // device clock
   uint32_t depth, bu_levels;
   uint32_t team, err;
-  uint32_t pad[4];
+  uint64_t edges;  // CSR entries of the reached vertices
+  uint32_t pad[2];
 };
 
 // Host-mapped status the kernel reports once it runs.
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.bottom_up = ((d.flags & kServeDirOpt) && g.symmetric) ? 1u : 0u;
     a.alpha = d.alpha;
     a.bu_exit = d.beta ? g.n / d.beta : 0;
+    a.count_edges = 1;
     a.m = g.m;
     grid_bfs<BLOCK>(a, sh, team);
     // outputs reach memory the host's copies read before the result appears
@@ -236,6 +238,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
       o->reached = a.ctl->reached;
       o->depth = a.ctl->depth;
       o->bu_levels = a.ctl->bu_levels;
+      o->edges = a.ctl->edges;
       o->team = id;
       o->err = a.ctl->err;
       o->t_start_ns = t0;

@@ -579,14 +579,14 @@ class ServerInfo(C.Structure):
 
 class ServeResult(C.Structure):
     _fields_ = [("graph", C.c_uint32), ("source", C.c_uint32), ("depth", C.c_uint32),
-                ("bottom_up_levels", C.c_uint32), ("reached", C.c_uint64), ("team", C.c_uint32),
-                ("reserved", C.c_uint32), ("service_us", C.c_double)]
+                ("bottom_up_levels", C.c_uint32), ("reached", C.c_uint64), ("edges", C.c_uint64),
+                ("team", C.c_uint32), ("reserved", C.c_uint32), ("service_us", C.c_double)]
 
 
 class ServeRecord(C.Structure):
     _fields_ = [("arrival_us", C.c_double), ("admit_us", C.c_double), ("done_us", C.c_double),
-                ("service_us", C.c_double), ("reached", C.c_uint64), ("depth", C.c_uint32),
-                ("team", C.c_uint32)]
+                ("service_us", C.c_double), ("reached", C.c_uint64), ("edges", C.c_uint64),
+                ("depth", C.c_uint32), ("team", C.c_uint32)]
 
 
 _sig("dfq_serve_config_default", C.c_int, C.POINTER(ServeConfig))
@@ -728,6 +728,6 @@ class Server:
         check(lib.dfq_server_replay(self._h, nq, _u32(gi), _u32(so), ar.ctypes.data_as(C.POINTER(C.c_double)),
                                     flags, out_ptrs, recs), "dfq_server_replay")
         dt = np.dtype([("arrival_us", "f8"), ("admit_us", "f8"), ("done_us", "f8"), ("service_us", "f8"),
-                       ("reached", "u8"), ("depth", "u4"), ("team", "u4")])
+                       ("reached", "u8"), ("edges", "u8"), ("depth", "u4"), ("team", "u4")])
         arr = np.frombuffer(bytearray(recs), dtype=dt, count=nq).copy() if nq else np.zeros(0, dt)
         return arr, outs

@@ -64,6 +64,8 @@ typedef struct dfq_serve_result {
   uint32_t depth;           /* levels (max level + 1)                            */
   uint32_t bottom_up_levels;
   uint64_t reached;
+  uint64_t edges;           /* CSR entries of the reached vertices (sum of their
+                               out-degrees): the edges the search traversed      */
   uint32_t team;            /* team that ran it                                  */
   uint32_t reserved;
   double   service_us;      /* device time from the team taking it to its result */
@@ -77,6 +79,7 @@ typedef struct dfq_serve_record {
   double   done_us;         /* completion seen by the host                       */
   double   service_us;      /* as in dfq_serve_result                            */
   uint64_t reached;
+  uint64_t edges;           /* as in dfq_serve_result                            */
   uint32_t depth;
   uint32_t team;
 } dfq_serve_record;

@@ -39,9 +39,11 @@ def serve_setup(dfq, oracle):
     return gs, gi, so, arrival, expect
 
 
-def check_record(rec, e):
+def check_record(rec, e, g):
     assert rec["reached"] == int((e != UNREACHED).sum())
     assert rec["depth"] == int(e[e != UNREACHED].max()) + 1
+    # traversed edges: out-degree sum of the reached vertices
+    assert rec["edges"] == int(g.degrees()[e != UNREACHED].sum())
 
 
 def max_overlap(recs):
@@ -66,7 +68,7 @@ def test_serve_mix_equals_alone_and_oracle(dfq, serve_setup):
         for q in range(len(gi)):
             e = expect[(gi[q], so[q])]
             np.testing.assert_array_equal(outs[q], e, err_msg=f"mix: query {q} graph {gi[q]} source {so[q]}")
-            check_record(recs[q], e)
+            check_record(recs[q], e, gs[gi[q]])
         assert (recs["done_us"] >= recs["admit_us"]).all() and (recs["admit_us"] >= recs["arrival_us"]).all()
         say("max in flight", max_overlap(recs), "teams used", sorted(set(recs["team"].tolist())))
         if info["teams"] > 1:
@@ -96,7 +98,7 @@ def test_serve_team_shapes(dfq, serve_setup, xpt, block):
         for q in range(len(recs)):
             e = expect[(gi[q], so[q])]
             np.testing.assert_array_equal(outs[q], e, err_msg=f"xpt {xpt}: query {q} graph {gi[q]}")
-            check_record(recs[q], e)
+            check_record(recs[q], e, gs[gi[q]])
         srv.stop()
 
 

@@ -56,6 +56,8 @@ struct dfq_batch {
   uint32_t *d_work = nullptr;
   uint32_t *d_wg_queue = nullptr;
   int wg_queue_grid = 0;
+  uint32_t *d_iso = nullptr;  // isolated-vertex bitmaps of the symmetric graphs
+  uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs' ranges)
   // launch state of the current tables
   BatchArgs args{};
   bool tables_ok = false;
@@ -89,7 +91,7 @@ void free_all(dfq_batch *b) {
   drop_graph_exec(b);
   void *ptrs[] = {b->d_off, b->d_col, b->d_graphs, b->d_level, b->d_queries, b->d_results,
                   b->d_vis, b->d_vis_snap, b->d_fb, b->d_qseg[0], b->d_qseg[1], b->d_items,
-                  b->d_ctl, b->d_work, b->d_wg_queue};
+                  b->d_ctl, b->d_work, b->d_wg_queue, b->d_iso, b->d_col_bu};
   for (void *p : ptrs) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -192,6 +194,8 @@ int build_tables(dfq_batch *b, const std::vector<uint32_t> &subset) {
   a.n_grid = (uint32_t)grid_q.size();
   a.work = b->d_work;
   a.wg_queue = b->d_wg_queue;
+  a.iso = b->d_iso;
+  a.col_bu = b->d_col_bu;
   BfsArgs &s = a.grid;
   s.vis = b->d_vis;
   s.vis_snap = b->d_vis_snap;
@@ -313,6 +317,11 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
     if (bad) return DFQ_ERR_RANGE;
     graphs[g].symmetric = is_symmetric(off, col, n[g]) ? 1u : 0u;
   }
+  uint64_t iso_words = 0;
+  for (auto &gr : graphs) {
+    gr.iso_base = gr.symmetric ? iso_words : kNoIso;
+    if (gr.symmetric) iso_words += ((uint64_t)gr.n + 31) / 32;
+  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DFQ_ERR_NO_DEVICE;
 
@@ -338,16 +347,36 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
   BFAIL(hipEventCreate(&b->ev0));
   BFAIL(hipEventCreate(&b->ev1));
   BFAIL(hipMalloc(&b->d_off, sizeof(int64_t) * vtot));
-  BFAIL(hipMalloc(&b->d_col, sizeof(uint32_t) * (etot ? etot : 1)));
+  // +4 entries: bottom-up levels read columns as aligned 16-byte blocks
+  BFAIL(hipMalloc(&b->d_col, sizeof(uint32_t) * (etot + 4)));
+  BFAIL(hipMalloc(&b->d_col_bu, sizeof(uint32_t) * (etot + 4)));
   BFAIL(hipMalloc(&b->d_graphs, sizeof(BatchGraph) * num_graphs));
   BFAIL(hipMemcpy(b->d_off, row_offsets, sizeof(int64_t) * vtot, hipMemcpyHostToDevice));
-  if (etot) BFAIL(hipMemcpy(b->d_col, col_idx, sizeof(uint32_t) * etot, hipMemcpyHostToDevice));
+  {
+    // symmetric graphs: a second copy of the rows by neighbour degree for
+    // bottom-up levels, and isolated vertices marked visited from the start
+    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+      const BatchGraph &gr = graphs[g];
+      const int64_t *off = row_offsets + gr.vbase;
+      const uint32_t *col = col_idx + gr.ebase;
+      if (gr.m) BFAIL(hipMemcpy(b->d_col + gr.ebase, col, sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+      if (!gr.symmetric) continue;
+      ordered.resize(gr.m);
+      order_rows_by_degree(off, col, gr.n, ordered.data());
+      isolated_bitmap(off, gr.n, iso.data() + gr.iso_base);
+      if (gr.m)
+        BFAIL(hipMemcpy(b->d_col_bu + gr.ebase, ordered.data(), sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+    }
+    BFAIL(hipMalloc(&b->d_iso, sizeof(uint32_t) * iso.size()));
+    BFAIL(hipMemcpy(b->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));
+  }
   BFAIL(hipMemcpy(b->d_graphs, graphs.data(), sizeof(BatchGraph) * num_graphs, hipMemcpyHostToDevice));
   // grid-tier scratch sized for the largest graph
   const uint32_t nw = (b->n_max + 31) / 32;
   BFAIL(hipMalloc(&b->d_vis, sizeof(uint32_t) * nw));
   BFAIL(hipMalloc(&b->d_vis_snap, sizeof(uint32_t) * nw));
-  BFAIL(hipMalloc(&b->d_fb, sizeof(uint32_t) * ((size_t)nw + 1)));
+  BFAIL(hipMalloc(&b->d_fb, sizeof(uint32_t) * 2 * ((size_t)nw + 1)));
   for (int i = 0; i < 2; ++i) BFAIL(hipMalloc(&b->d_qseg[i], sizeof(uint32_t) * (size_t)kSegments * b->n_max));
   b->item_cap = (uint32_t)std::min<uint64_t>(b->m_max / kItem + b->n_max, 0xffffffffull);
   BFAIL(hipMalloc(&b->d_items, sizeof(uint2) * (size_t)kSegments * b->item_cap));

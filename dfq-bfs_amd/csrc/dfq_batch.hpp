@@ -30,7 +30,10 @@ struct BatchGraph {
   uint64_t m;          // CSR entries
   uint32_t n;          // vertices
   uint32_t symmetric;  // bottom-up levels allowed
+  uint64_t iso_base;   // first word of its isolated-vertex bitmap, kNoIso if none
 };
+
+constexpr uint64_t kNoIso = ~0ull;
 
 struct BatchQuery {
   uint64_t lbase;      // first entry of this query's level array in the output
@@ -51,6 +54,7 @@ struct BatchResult {
 struct BatchArgs {
   const int64_t *off;         // concatenated row offsets
   const uint32_t *col;        // concatenated columns
+  const uint32_t *col_bu;     // the same, rows by neighbour degree (symmetric graphs)
   const BatchGraph *graphs;
   const BatchQuery *queries;  // grid-tier queries first, then workgroup tier
   BatchResult *results;
@@ -59,6 +63,7 @@ struct BatchArgs {
   uint32_t n_grid;            // queries [0, n_grid) are grid tier
   uint32_t *work;             // workgroup-tier instance counter
   uint32_t *wg_queue;         // per-workgroup frontier partitions [grid][2][kWgMaxN]
+  const uint32_t *iso;        // concatenated isolated-vertex bitmaps (symmetric graphs)
   BfsArgs grid;               // scratch of the grid tier (sized for its largest graph)
 };
 
@@ -179,9 +184,9 @@ __device__ __forceinline__ void wg_expand_rows(WgVisitor &vis, const uint32_t *q
 
 // One BFS by one workgroup.  off / col point at the instance's graph.
 template <int BLOCK>
-__device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *col, uint32_t n,
-                       bool symmetric, const BatchQuery &q, uint32_t *level, uint32_t *qbuf,
-                       BatchResult *res) {
+__device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *col, const uint32_t *col_bu,
+                       uint32_t n, bool symmetric, const uint32_t *iso, const BatchQuery &q, uint32_t *level,
+                       uint32_t *qbuf, BatchResult *res) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
@@ -190,7 +195,7 @@ __device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *
   for (uint32_t i = tid; i < n; i += BLOCK) level[i] = (i == src) ? 0u : kInf;
   for (uint32_t w = tid; w < words; w += BLOCK) {
     const uint32_t init = (w == (src >> 5)) ? (1u << (src & 31u)) : 0u;
-    sh.vis[w] = init;
+    sh.vis[w] = init | (iso ? iso[w] : 0u);
     sh.fb[0][w] = init;
     sh.fb[1][w] = 0;
   }
@@ -302,7 +307,7 @@ __device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *
           for (int64_t k = s; k < e && !claimed; k += kUnroll) {
             uint32_t w[kUnroll];
 #pragma unroll
-            for (int i = 0; i < kUnroll; ++i) w[i] = k + i < e ? col[k + i] : 0xffffffffu;
+            for (int i = 0; i < kUnroll; ++i) w[i] = k + i < e ? col_bu[k + i] : 0xffffffffu;
 #pragma unroll
             for (int i = 0; i < kUnroll; ++i)
               if (w[i] != 0xffffffffu && ((fb_cur[w[i] >> 5] >> (w[i] & 31u)) & 1u)) claimed = true;
@@ -374,11 +379,13 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     BfsArgs a = b.grid;
     a.off = b.off + gr.vbase;
     a.col = b.col + gr.ebase;
+    a.col_bu = gr.symmetric ? b.col_bu + gr.ebase : nullptr;
     a.level = b.level + q.lbase;
     a.parent = nullptr;
     a.n = gr.n;
     a.n_words = (gr.n + 31) / 32;
     a.m = gr.m;
+    a.iso = gr.iso_base != kNoIso ? b.iso + gr.iso_base : nullptr;
     a.source = q.source;
     a.small_max = q.small_max;
     a.medium_max = q.medium_max;
@@ -405,7 +412,9 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     if (qi >= b.n_queries) break;
     const BatchQuery q = b.queries[qi];
     const BatchGraph gr = b.graphs[q.graph];
-    wg_bfs<BLOCK>(sh.wg, b.off + gr.vbase, b.col + gr.ebase, gr.n, gr.symmetric != 0, q,
+    wg_bfs<BLOCK>(sh.wg, b.off + gr.vbase, b.col + gr.ebase,
+                  (gr.symmetric ? b.col_bu : b.col) + gr.ebase, gr.n, gr.symmetric != 0,
+                  gr.iso_base != kNoIso ? b.iso + gr.iso_base : nullptr, q,
                   b.level + q.lbase, qbuf, &b.results[qi]);
   }
 }

@@ -62,6 +62,8 @@ struct dfq_graph {
   uint32_t *d_vis = nullptr;
   uint32_t *d_vis_snap = nullptr;
   uint32_t *d_fb = nullptr;
+  uint32_t *d_iso = nullptr;     // isolated vertices (symmetric graphs only)
+  uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs only)
   uint2 *d_items = nullptr;
   uint32_t item_cap = 0;
   uint32_t *d_qseg[2] = {nullptr, nullptr};  // kSegments x n entries each
@@ -120,6 +122,8 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_vis);
   (void)hipFree(g->d_vis_snap);
   (void)hipFree(g->d_fb);
+  (void)hipFree(g->d_iso);
+  (void)hipFree(g->d_col_bu);
   (void)hipFree(g->d_items);
   (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
@@ -137,6 +141,8 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.vis = g->d_vis;
   a.vis_snap = g->d_vis_snap;
   a.fb = g->d_fb;
+  a.iso = g->d_iso;
+  a.col_bu = g->d_col_bu;
   a.qseg[0] = g->d_qseg[0];
   a.qseg[1] = g->d_qseg[1];
   a.items = g->d_items;
@@ -275,12 +281,13 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   HIP_OR_FAIL(hipEventCreate(&g->ev0));
   HIP_OR_FAIL(hipEventCreate(&g->ev1));
   HIP_OR_FAIL(hipMalloc(&g->d_off, sizeof(int64_t) * ((size_t)n + 1)));
-  HIP_OR_FAIL(hipMalloc(&g->d_col, sizeof(uint32_t) * (m ? m : 1)));
+  // +4 entries: bottom-up levels read columns as aligned 16-byte blocks
+  HIP_OR_FAIL(hipMalloc(&g->d_col, sizeof(uint32_t) * (m + 4)));
   HIP_OR_FAIL(hipMalloc(&g->d_level, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_parent, sizeof(uint32_t) * n));
   HIP_OR_FAIL(hipMalloc(&g->d_vis, sizeof(uint32_t) * g->n_words));
   HIP_OR_FAIL(hipMalloc(&g->d_vis_snap, sizeof(uint32_t) * g->n_words));
-  HIP_OR_FAIL(hipMalloc(&g->d_fb, sizeof(uint32_t) * ((size_t)g->n_words + 1)));
+  HIP_OR_FAIL(hipMalloc(&g->d_fb, sizeof(uint32_t) * 2 * ((size_t)g->n_words + 1)));
   // one level's items: sum over frontier vertices of ceil(deg / kItem)
   // <= m / kItem + n; any segment may receive all of them
   g->item_cap = (uint32_t)std::min<uint64_t>(m / kItem + n, 0xffffffffull);
@@ -290,6 +297,25 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   HIP_OR_FAIL(hipMalloc(&g->d_ctl, sizeof(Control)));
   HIP_OR_FAIL(hipMemcpy(g->d_off, row_offsets, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
   if (m) HIP_OR_FAIL(hipMemcpy(g->d_col, col_idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+  if (g->symmetric) {
+    // bottom-up levels read a second copy of the rows ordered by neighbour
+    // degree (top-down levels keep the input order: degree order puts the
+    // same hubs at the front of every row, and the probes of all waves then
+    // land on the same bitmap lines at once); isolated vertices start visited
+    std::vector<uint32_t> ordered;
+    std::vector<uint32_t> iso(g->n_words);
+    try {
+      ordered.resize(m);
+    } catch (...) {
+      return fail(DFQ_ERR_NOMEM);
+    }
+    order_rows_by_degree(row_offsets, col_idx, n, ordered.data());
+    isolated_bitmap(row_offsets, n, iso.data());
+    HIP_OR_FAIL(hipMalloc(&g->d_col_bu, sizeof(uint32_t) * (m + 4)));
+    if (m) HIP_OR_FAIL(hipMemcpy(g->d_col_bu, ordered.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMalloc(&g->d_iso, sizeof(uint32_t) * g->n_words));
+    HIP_OR_FAIL(hipMemcpy(g->d_iso, iso.data(), sizeof(uint32_t) * g->n_words, hipMemcpyHostToDevice));
+  }
   HIP_OR_FAIL(hipMemset(g->d_ctl, 0, sizeof(Control)));
 #undef HIP_OR_FAIL
   dfq_config cfg;

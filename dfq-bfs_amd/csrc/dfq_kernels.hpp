@@ -83,12 +83,14 @@ __device__ __forceinline__ uint64_t now_ns() { return __builtin_amdgcn_s_memreal
 
 struct BfsArgs {
   const int64_t *off;    // row offsets [n+1]
-  const uint32_t *col;   // column indices [m]
+  const uint32_t *col;   // column indices [m] (top-down levels read these)
+  const uint32_t *col_bu; // the same rows ordered by neighbour degree (bottom-up levels), or null
   uint32_t *level;       // [n]
   uint32_t *parent;      // [n] or null
+  const uint32_t *iso;   // isolated vertices [n_words] (symmetric graphs) or null
   uint32_t *vis;         // visited bitmap [n_words]
   uint32_t *vis_snap;    // its copy taken at each top-down level's start [n_words]
-  uint32_t *fb;          // frontier bitmap [n_words + 1] (bottom-up levels)
+  uint32_t *fb;          // two frontier bitmaps [2][n_words + 1] (bottom-up levels)
   uint32_t *qseg[2];     // frontier queues: kSegments segments of n entries each
   uint2 *items;          // kSegments segments of item_cap work items
   uint32_t item_cap;     // items per segment (m / kItem + n bounds one level)
@@ -132,11 +134,18 @@ struct SharedA {
   int64_t g_s[KernelShape<BLOCK>::kWaves][kWave];
 };
 
+// Bottom-up levels work on chunks of kBuWords visited words (kBuChunk
+// vertices) per wave.
+constexpr uint32_t kBuWords = 32;
+constexpr uint32_t kBuChunk = kBuWords * 32;
+
 template <int BLOCK>
 struct SharedAll {
-  struct {
-    SharedA<BLOCK> a;
+  union {
+    SharedA<BLOCK> a;                                       // top-down bins
+    uint32_t cand[KernelShape<BLOCK>::kWaves][kBuChunk];   // bottom-up candidates
   } u;
+  uint32_t bu_claim[KernelShape<BLOCK>::kWaves][kBuWords];  // bottom-up claims of the chunk
   uint32_t stage[KernelShape<BLOCK>::kWaves][kStage];
   uint32_t stage_n[KernelShape<BLOCK>::kWaves];
   uint32_t stage_base[KernelShape<BLOCK>::kWaves];
@@ -442,65 +451,143 @@ __device__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
   }
 }
 
-// Bottom-up pass 1: frontier bitmap fb = { v : level[v] == L } (one 64-vertex
-// group per wave step, coalesced level reads).
-__device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t L, uint32_t gwave,
-                                                      uint32_t nwaves) {
+// Frontier bitmap of a bottom-up level that follows a top-down one:
+// fb = { v : level[v] == L }, one 64-vertex group per wave step, eight
+// groups' level loads in flight at once.
+__device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t L, uint32_t *fb,
+                                                      uint32_t gwave, uint32_t nwaves) {
+  constexpr int B = 8;
   const uint32_t groups = (a.n + 63) / 64;
-  for (uint32_t g = gwave; g < groups; g += nwaves) {
-    const uint32_t v = g * 64 + lane_id();
-    const bool f = v < a.n && a.level[v] == L;
-    const uint64_t m = __ballot(f);
-    if (lane_id() == 0) a.fb[2 * g] = (uint32_t)m;
-    if (lane_id() == 1 && 2 * g + 1 < a.n_words + 1) a.fb[2 * g + 1] = (uint32_t)(m >> 32);
+  const uint32_t lane = lane_id();
+  for (uint32_t g0 = gwave; g0 < groups; g0 += B * nwaves) {
+    uint32_t lv[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const uint32_t v = (g0 + i * nwaves) * 64 + lane;
+      lv[i] = (g0 + i * nwaves < groups && v < a.n) ? a.level[v] : kInf;
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const uint32_t g = g0 + i * nwaves;
+      if (g >= groups) break;  // wave-uniform
+      const uint64_t m = __ballot(lv[i] == L);
+      if (lane == 0) fb[2 * g] = (uint32_t)m;
+      if (lane == 1 && 2 * g + 1 < a.n_words + 1) fb[2 * g + 1] = (uint32_t)(m >> 32);
+    }
   }
 }
 
-// Bottom-up pass 2: every unvisited vertex scans its neighbours for one in
-// the frontier; the owning wave updates the visited words directly.
-__device__ __forceinline__ void bottomup_expand(const BfsArgs &a, Visitor &vis, uint32_t L,
-                                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
-  const uint32_t groups = (a.n + 63) / 64;
-  for (uint32_t g = gwave; g < groups; g += nwaves) {
-    const uint32_t v = g * 64 + lane_id();
-    const uint32_t w0 = a.vis[2 * g];
-    const uint32_t w1 = 2 * g + 1 < a.n_words ? a.vis[2 * g + 1] : 0xffffffffu;
-    const uint64_t seen = ((uint64_t)w1 << 32) | w0;
-    bool claimed = false;
-    uint32_t par = 0;
-    if (v < a.n && !((seen >> lane_id()) & 1ull)) {
-      const int64_t s = a.off[v], e = a.off[v + 1];
-      // kUnroll neighbours per step: their frontier-bitmap words are fetched
-      // together, and the first neighbour in the frontier (in row order)
-      // becomes the parent
-      for (int64_t k = s; k < e && !claimed; k += kUnroll) {
-        uint32_t w[kUnroll], f[kUnroll];
-#pragma unroll
-        for (int i = 0; i < kUnroll; ++i) w[i] = k + i < e ? a.col[k + i] : 0u;
-#pragma unroll
-        for (int i = 0; i < kUnroll; ++i) f[i] = k + i < e ? a.fb[w[i] >> 5] : 0u;
-#pragma unroll
-        for (int i = kUnroll - 1; i >= 0; --i)
-          if ((f[i] >> (w[i] & 31u)) & 1u) {
-            claimed = true;
-            par = w[i];
-          }
-        const int64_t scanned = min((int64_t)kUnroll, e - k);
-        vis.probes += (uint32_t)scanned;
-        edges += (uint64_t)scanned;
+// One bottom-up level.  Each wave takes chunks of kBuWords visited words:
+// it lists the chunk's unvisited vertices in LDS, then every lane streams
+// through its share of that list as a three-stage pipeline — row offsets,
+// kUnroll column indices, their frontier-bitmap words — so that each loop
+// trip costs one memory round trip for the whole wave whatever stage each
+// lane is in, and a lane that finishes a vertex starts the next at once.
+// The first frontier neighbour in row order becomes the parent.  Claims are
+// collected in LDS; the wave then writes the chunk's visited words and the
+// next level's frontier-bitmap words itself (it owns them: no atomics).
+template <int BLOCK>
+__device__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t L,
+                               const uint32_t *fb_cur, uint32_t *fb_nxt, uint32_t wave,
+                               uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
+  const uint32_t lane = lane_id();
+  const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
+  uint32_t *cand = sh.u.cand[wave];
+  uint32_t *cm = sh.bu_claim[wave];
+  const uint32_t chunks = (a.n_words + kBuWords - 1) / kBuWords;
+  for (uint32_t c = gwave; c < chunks; c += nwaves) {
+    const uint32_t w0 = c * kBuWords;
+    const uint32_t wi = w0 + lane;
+    const bool own = lane < kBuWords && wi < a.n_words;
+    const uint32_t word = own ? a.vis[wi] : 0xffffffffu;
+    uint32_t unv = ~word;
+    if (own && wi == a.n_words - 1 && (a.n & 31u)) unv &= (1u << (a.n & 31u)) - 1u;  // no vertex past n
+    const uint32_t cnt = (uint32_t)__popc(unv);
+    const uint32_t incl = wave_inclusive_scan(cnt);
+    const uint32_t total = bcast_lane(incl, 63);
+    if (lane < kBuWords) cm[lane] = 0;
+    for (uint32_t pos = incl - cnt; unv; unv &= unv - 1u) cand[pos++] = (wi << 5) + (uint32_t)(__ffs(unv) - 1);
+    wave_sync();
+
+    // Lanes that need a vertex take the next ones of the list in order, so
+    // the row-offset loads of one trip fall on neighbouring vertices.
+    // Column indices come as one aligned 16-byte load per step (entries of
+    // the block outside [k, e) are ignored), so a step is one request.
+    // Every lane issues every load of a trip from straight-line code; lanes
+    // with nothing to load for a kind read one shared address (the wave
+    // merges them into one request), so the trip waits on memory once.
+    uint32_t cursor = 0, stage = 0, v = 0;
+    int64_t s0 = 0, k = 0, e = 0, kb = 0;
+    uint32_t cw[4] = {0u, 0u, 0u, 0u};
+    for (;;) {
+      const uint64_t need = __ballot(stage == 0);
+      const uint32_t ci = cursor + mbcnt(need);
+      cursor += (uint32_t)__popcll(need);
+      if (stage == 0) {
+        if (ci < total) v = cand[ci];
+        else stage = 3;
       }
+      if (!__ballot(stage != 3)) break;
+      const bool st0 = stage == 0, st1 = stage == 1, st2 = stage == 2;
+      const int64_t *po = a.off + (st0 ? v : 0u);
+      const uint32_t *pk = colb + (st1 ? k : 0);
+      const uint32_t back = (uint32_t)(((uintptr_t)pk & 15) >> 2);
+      uint32_t fi[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) fi[i] = (st2 && kb + i >= k && kb + i < e) ? (cw[i] >> 5) : 0u;
+      const int64_t o0 = po[0], o1 = po[1];
+      const uint4 q = *(const uint4 *)(pk - back);  // derived from col: a global load
+      uint32_t fw[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) fw[i] = fb_cur[fi[i]];
+      // next state by selects only: a branch here would let the compiler
+      // sink each load into the branch that uses it, one wait per branch
+      bool hit = false;
+      uint32_t par = 0;
+#pragma unroll
+      for (int i = 3; i >= 0; --i) {
+        const bool h = (kb + i >= k) & (kb + i < e) & (((fw[i] >> (cw[i] & 31u)) & 1u) != 0u);
+        hit = hit || h;
+        par = h ? cw[i] : par;
+      }
+      const int64_t kn = kb + 4;
+      const bool claimed = st2 && hit;
+      const uint32_t scanned = st2 ? (uint32_t)(min(kn, e) - k) : 0u;
+      vis.probes += scanned;
+      edges += scanned;
+      const uint32_t next2 = (!hit && kn < e) ? 1u : 0u;
+      const uint32_t next0 = o1 > o0 ? 1u : 0u;
+      stage = st0 ? next0 : (st1 ? 2u : (st2 ? next2 : stage));
+      cw[0] = st1 ? q.x : cw[0];
+      cw[1] = st1 ? q.y : cw[1];
+      cw[2] = st1 ? q.z : cw[2];
+      cw[3] = st1 ? q.w : cw[3];
+      kb = st1 ? k - (int64_t)back : kb;  // CSR index of cw[0]
+      const int64_t deg = e - s0;
+      k = st0 ? o0 : (st2 ? kn : k);
+      s0 = st0 ? o0 : s0;
+      e = st0 ? o1 : e;
+      if (claimed) {
+        if (a.parent) a.parent[v] = par;
+        vis.mf += (uint64_t)deg;
+        atomicOr(&cm[(v >> 5) - w0], 1u << (v & 31u));
+      }
+      const uint64_t mask = __ballot(claimed);
+      if (mask) vis.append(claimed, mask, v);
     }
-    if (claimed) {
-      a.level[v] = L + 1;
-      if (a.parent) a.parent[v] = par;
-      vis.mf += (uint64_t)(a.off[v + 1] - a.off[v]);
+    wave_sync();
+    if (own) {
+      const uint32_t got = cm[lane];
+      if (got) a.vis[wi] = word | got;
+      fb_nxt[wi] = got;
     }
-    const uint64_t mask = __ballot(claimed);
-    if (mask) {
-      if (lane_id() == 0) a.vis[2 * g] = w0 | (uint32_t)mask;
-      if (lane_id() == 1 && 2 * g + 1 < a.n_words) a.vis[2 * g + 1] = w1 | (uint32_t)(mask >> 32);
-      vis.append(claimed, mask, v);
+    // levels of the chunk's claims, 64 consecutive vertices per store
+    // instruction (a few requests instead of one per claim)
+    for (uint32_t p = 0; p < kBuWords / 2; ++p) {
+      const uint32_t bits = cm[2 * p + (lane >> 5)];
+      if ((bits >> (lane & 31u)) & 1u) a.level[(w0 + 2 * p) * 32 + lane] = L + 1;
     }
+    wave_sync();
   }
 }
 
@@ -527,7 +614,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
     if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
   }
   for (uint32_t w = gtid; w < a.n_words; w += gstride) {
-    const uint32_t init = (w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u;
+    const uint32_t init = ((w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u) | (a.iso ? a.iso[w] : 0u);
     a.vis[w] = init;
     a.vis_snap[w] = init;
   }
@@ -566,7 +653,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
   uint64_t m_f = (uint64_t)(a.off[a.source + 1] - a.off[a.source]);
   uint64_t m_seen = m_f;
   uint32_t bu_levels = 0;
-  bool bottom_up = false;
+  bool bottom_up = false, prev_bottom_up = false;
   for (uint32_t L = 0;; ++L) {
     const uint32_t *in_seg = a.qseg[L & 1];
     const LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
@@ -626,11 +713,19 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
       }
     } else {
       ++bu_levels;
-      build_frontier_bitmap(a, L, gwave, nwaves);
-      grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+      // frontier bitmaps alternate by level parity; a bottom-up level writes
+      // the next level's, so only the first after a top-down level scans
+      // the level array
+      const size_t fbs = (size_t)a.n_words + 1;
+      uint32_t *fb_cur = a.fb + (L & 1) * fbs;
+      uint32_t *fb_nxt = a.fb + ((L + 1) & 1) * fbs;
+      if (!prev_bottom_up) {
+        build_frontier_bitmap(a, L, fb_cur, gwave, nwaves);
+        grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+      }
       uint64_t tb = 0;
       if (a.prof && tid == 0) tb = now_ns();
-      bottomup_expand(a, vis, L, gwave, nwaves, edges);
+      bottomup_level<BLOCK>(a, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
       flush_block<BLOCK>(vis, sh, wave);
       if (a.prof && tid == 0) busy += now_ns() - tb;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
@@ -639,6 +734,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
       m_f = mf1;
     }
     m_seen += m_f;
+    prev_bottom_up = bottom_up;
     if (a.prof) {
       prof_add<BLOCK>(sh, kProfEdges, edges);
       prof_add<BLOCK>(sh, kProfProbes, vis.probes);

@@ -125,6 +125,8 @@ struct dfq_server {
   int64_t *d_off = nullptr;
   uint32_t *d_col = nullptr;
   ServeGraph *d_graphs = nullptr;
+  uint32_t *d_iso = nullptr;
+  uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs' ranges)
   ServeCtl *d_ctl = nullptr;
   uint8_t *d_arena = nullptr;
   // pinned, host-mapped
@@ -186,7 +188,7 @@ bool kernel_ended(dfq_server *s) { return hipStreamQuery(s->stream) != hipErrorN
 
 void free_all(dfq_server *s) {
   if (s->wedged) return;  // a launch may still read this memory
-  void *dev[] = {s->d_off, s->d_col, s->d_graphs, s->d_ctl, s->d_arena};
+  void *dev[] = {s->d_off, s->d_col, s->d_col_bu, s->d_graphs, s->d_iso, s->d_ctl, s->d_arena};
   for (void *p : dev) (void)hipFree(p);
   if (s->h_block) (void)hipHostFree(s->h_block);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -356,6 +358,11 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
     if (!csr_valid(off, col, n[g], m[g])) return DFQ_ERR_RANGE;
     graphs[g].symmetric = is_symmetric(off, col, n[g]) ? 1u : 0u;
   }
+  uint64_t iso_words = 0;
+  for (auto &gr : graphs) {
+    gr.iso_base = gr.symmetric ? iso_words : ~0ull;
+    if (gr.symmetric) iso_words += ((uint64_t)gr.n + 31) / 32;
+  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DFQ_ERR_NO_DEVICE;
 
@@ -379,10 +386,30 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   s->plans.assign(num_graphs, default_serve_plan());
   SFAIL(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   SFAIL(hipMalloc(&s->d_off, sizeof(int64_t) * vtot));
-  SFAIL(hipMalloc(&s->d_col, sizeof(uint32_t) * (etot ? etot : 1)));
+  // +4 entries: bottom-up levels read columns as aligned 16-byte blocks
+  SFAIL(hipMalloc(&s->d_col, sizeof(uint32_t) * (etot + 4)));
+  SFAIL(hipMalloc(&s->d_col_bu, sizeof(uint32_t) * (etot + 4)));
   SFAIL(hipMalloc(&s->d_graphs, sizeof(ServeGraph) * num_graphs));
   SFAIL(hipMemcpy(s->d_off, row_offsets, sizeof(int64_t) * vtot, hipMemcpyHostToDevice));
-  if (etot) SFAIL(hipMemcpy(s->d_col, col_idx, sizeof(uint32_t) * etot, hipMemcpyHostToDevice));
+  {
+    // symmetric graphs: a second copy of the rows by neighbour degree for
+    // bottom-up levels, and isolated vertices marked visited from the start
+    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+      const ServeGraph &gr = graphs[g];
+      const int64_t *off = row_offsets + gr.vbase;
+      const uint32_t *col = col_idx + gr.ebase;
+      if (gr.m) SFAIL(hipMemcpy(s->d_col + gr.ebase, col, sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+      if (!gr.symmetric) continue;
+      ordered.resize(gr.m);
+      order_rows_by_degree(off, col, gr.n, ordered.data());
+      isolated_bitmap(off, gr.n, iso.data() + gr.iso_base);
+      if (gr.m)
+        SFAIL(hipMemcpy(s->d_col_bu + gr.ebase, ordered.data(), sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+    }
+    SFAIL(hipMalloc(&s->d_iso, sizeof(uint32_t) * iso.size()));
+    SFAIL(hipMemcpy(s->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));
+  }
   SFAIL(hipMemcpy(s->d_graphs, graphs.data(), sizeof(ServeGraph) * num_graphs, hipMemcpyHostToDevice));
   SFAIL(hipMalloc(&s->d_ctl, sizeof(ServeCtl)));
   SFAIL(hipMemset(s->d_ctl, 0, sizeof(ServeCtl)));
@@ -423,6 +450,8 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   a.off = s->d_off;
   a.col = s->d_col;
   a.graphs = s->d_graphs;
+  a.iso = s->d_iso;
+  a.col_bu = s->d_col_bu;
   a.req = (ServeRequest *)db;
   a.res = (ServeResult *)(db + sizeof(ServeRequest) * R);
   a.stop = (const uint32_t *)(db + sizeof(ServeRequest) * R + sizeof(ServeResult) * R);

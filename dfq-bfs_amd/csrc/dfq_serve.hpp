@@ -26,6 +26,7 @@ struct ServeGraph {
   uint64_t m;
   uint32_t n;
   uint32_t symmetric;
+  uint64_t iso_base;   // first word of its isolated-vertex bitmap, ~0 if none
 };
 
 // Written by the host, read by the team leader.  seq = ticket + 1 publishes.
@@ -101,7 +102,7 @@ __host__ __device__ inline ServeLayout serve_layout(uint32_t n, uint32_t item_ca
   l.parent = o; if (parents) o += serve_align(4ull * n);
   l.vis = o;    o += serve_align(4 * words);
   l.snap = o;   o += serve_align(4 * words);
-  l.fb = o;     o += serve_align(4 * (words + 1));
+  l.fb = o;     o += serve_align(8 * (words + 1));
   l.qseg0 = o;  o += serve_align(4ull * segs * n);
   l.qseg1 = o;  o += serve_align(4ull * segs * n);
   l.items = o;  o += serve_align(8ull * segs * item_cap);
@@ -112,7 +113,9 @@ __host__ __device__ inline ServeLayout serve_layout(uint32_t n, uint32_t item_ca
 struct ServeArgs {
   const int64_t *off;           // concatenated row offsets
   const uint32_t *col;          // concatenated columns
+  const uint32_t *col_bu;       // the same, rows by neighbour degree (symmetric graphs)
   const ServeGraph *graphs;
+  const uint32_t *iso;          // concatenated isolated-vertex bitmaps (symmetric graphs)
   ServeRequest *req;            // host-mapped ring [ring]
   ServeResult *res;             // host-mapped ring [ring]
   const uint32_t *stop;         // host-mapped: set after the host has drained
@@ -206,8 +209,10 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     BfsArgs a;
     a.off = s.off + g.vbase;
     a.col = s.col + g.ebase;
+    a.col_bu = g.symmetric ? s.col_bu + g.ebase : nullptr;
     a.level = (uint32_t *)(w + l.level);
     a.parent = parents ? (uint32_t *)(w + l.parent) : nullptr;
+    a.iso = g.iso_base != ~0ull ? s.iso + g.iso_base : nullptr;
     a.vis = (uint32_t *)(w + l.vis);
     a.vis_snap = (uint32_t *)(w + l.snap);
     a.fb = (uint32_t *)(w + l.fb);

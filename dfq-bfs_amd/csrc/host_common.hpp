@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -123,6 +124,54 @@ inline bool is_symmetric(const int64_t *off, const uint32_t *col, uint32_t n) {
       }
   });
   return ok;
+}
+
+// Bottom-up levels stop at a vertex's first neighbour in the frontier, and
+// high-degree vertices join the frontier first, so rows are stored with
+// their neighbours by decreasing degree (ties by id): the same neighbour
+// set, found sooner (degree-ordered adjacency lists for bottom-up search as
+// in Yasui, Fujisawa and Goto, IEEE BigData 2013).  out has m entries.
+inline void order_rows_by_degree(const int64_t *off, const uint32_t *col, uint32_t n, uint32_t *out) {
+  std::vector<uint32_t> deg(n);
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t v = b; v < e; ++v)
+      deg[v] = (uint32_t)std::min<int64_t>(off[v + 1] - off[v], 0xffffffffll);
+  });
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    std::vector<uint64_t> key;
+    for (uint64_t u = b; u < e; ++u) {
+      const int64_t s = off[u], t = off[u + 1];
+      if (t - s < 2) {
+        if (t > s) out[s] = col[s];
+        continue;
+      }
+      key.resize((size_t)(t - s));
+      for (int64_t k = s; k < t; ++k) {
+        const uint32_t w = col[k];
+        key[(size_t)(k - s)] = ((uint64_t)(0xffffffffu - deg[w]) << 32) | w;
+      }
+      std::sort(key.begin(), key.end());
+      for (int64_t k = s; k < t; ++k) out[k] = (uint32_t)key[(size_t)(k - s)];
+    }
+  });
+}
+
+// Bit v set for every vertex without out-entries.  In a symmetric graph such
+// a vertex has no in-entries either, so no search reaches it: searches may
+// start with these bits already in the visited bitmap, and bottom-up levels
+// then never look at them.
+inline void isolated_bitmap(const int64_t *off, uint32_t n, uint32_t *words) {
+  const uint32_t nw = (n + 31) / 32;
+  parallel_for(nw, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t w = b; w < e; ++w) {
+      uint32_t bits = 0;
+      for (uint32_t i = 0; i < 32; ++i) {
+        const uint64_t v = w * 32 + i;
+        if (v < n && off[v + 1] == off[v]) bits |= 1u << i;
+      }
+      words[w] = bits;
+    }
+  });
 }
 
 }  // namespace dfq

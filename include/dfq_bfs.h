@@ -102,7 +102,8 @@ int dfq_graph_set_config(dfq_graph *g, const dfq_config *cfg);
 int dfq_graph_get_config(const dfq_graph *g, dfq_config *cfg);
 
 /* Device pointers of the resident CSR (for callers that keep other device
- * data next to it).  Valid until dfq_graph_destroy. */
+ * data next to it).  Valid until dfq_graph_destroy.  The columns are the
+ * input's, in the input's order. */
 int dfq_graph_device_arrays(const dfq_graph *g, const int64_t **d_row_offsets,
                             const uint32_t **d_col_idx);
 

@@ -84,7 +84,7 @@ def test_serve_config_and_footprint(dfq):
     # item capacity: m / 256 + min(n, m / (large_max + 1)) + 1
     cap = m // 256 + min(n, m // 257) + 1
     for xpt, segs in ((1, 1), (2, 2), (8, 8), (0, 8)):
-        expect = (al(4 * n) + 2 * al(4 * words) + al(4 * (words + 1)) + 2 * al(4 * segs * n)
+        expect = (al(4 * n) + 2 * al(4 * words) + al(8 * (words + 1)) + 2 * al(4 * segs * n)
                   + al(8 * segs * cap))
         assert dfq.serve_footprint(n, m, 256, xpt, 0) == expect
         assert dfq.serve_footprint(n, m, 256, xpt, dfq.SERVE_PARENTS) == expect + al(4 * n)

@@ -294,7 +294,7 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
 // frontier evenly over all workgroups.
 // ---------------------------------------------------------------------------
 constexpr int kSegments = 8;
-constexpr uint32_t kStage = 256;  // per-wave LDS staging entries
+constexpr uint32_t kStage = 1024;  // per-wave LDS staging entries
 
 // Visited-bitmap claim: a plain load filters vertices already seen (a stale
 // copy can only say "unvisited", never the reverse, because bits are only

@@ -184,7 +184,7 @@ __device__ __forceinline__ void wg_expand_rows(WgVisitor &vis, const uint32_t *q
 
 // One BFS by one workgroup.  off / col point at the instance's graph.
 template <int BLOCK>
-__device__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *col, const uint32_t *col_bu,
+__device__ __forceinline__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, const uint32_t *col, const uint32_t *col_bu,
                        uint32_t n, bool symmetric, const uint32_t *iso, const BatchQuery &q, uint32_t *level,
                        uint32_t *qbuf, BatchResult *res) {
   using S = KernelShape<BLOCK>;

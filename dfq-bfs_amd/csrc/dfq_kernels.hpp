@@ -139,11 +139,22 @@ struct SharedA {
 constexpr uint32_t kBuWords = 32;
 constexpr uint32_t kBuChunk = kBuWords * 32;
 
+// Small top-down frontier (at most one entry per thread): vertices, row
+// starts and the exclusive prefix of their degrees.
+template <int BLOCK>
+struct SharedSmall {
+  uint32_t v[BLOCK];
+  int64_t start[BLOCK];
+  uint64_t pre[BLOCK];
+  uint64_t wave_tot[KernelShape<BLOCK>::kWaves];
+};
+
 template <int BLOCK>
 struct SharedAll {
   union {
     SharedA<BLOCK> a;                                       // top-down bins
     uint32_t cand[KernelShape<BLOCK>::kWaves][kBuChunk];   // bottom-up candidates
+    SharedSmall<BLOCK> f;                                   // small frontiers
   } u;
   uint32_t bu_claim[KernelShape<BLOCK>::kWaves][kBuWords];  // bottom-up claims of the chunk
   uint32_t stage[KernelShape<BLOCK>::kWaves][kStage];
@@ -316,7 +327,7 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 
 // Top-down phase A over this workgroup's equal share of the frontier.
 template <int BLOCK>
-__device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
+__device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
                                 uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
   using S = KernelShape<BLOCK>;
@@ -422,10 +433,83 @@ __device__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor 
   expand_large<BLOCK>(vis, sh, bn);
 }
 
+// Top-down level of a frontier with at most BLOCK entries: every workgroup
+// reads the whole frontier and its degrees, and the team's waves split the
+// concatenated edge list evenly (edge-balanced, no bins, no work items, so
+// the level needs one barrier).  An edge's vertex is found by a binary
+// search of the degree prefix in LDS.
+template <int BLOCK>
+__device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, const uint32_t *in_seg,
+                              const LevelSlot &in_slot, uint32_t wave, uint32_t gwave, uint32_t nwaves,
+                              uint64_t &edges) {
+  constexpr int W = KernelShape<BLOCK>::kWaves;
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  if (tid == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kSegments; ++i) {
+      sh.seg_pre[i] = t;
+      t += ld_agent(&in_slot.tail[i]);
+    }
+    sh.seg_pre[kSegments] = t;
+  }
+  __syncthreads();
+  const uint32_t total = sh.seg_pre[kSegments];
+  uint32_t v = 0;
+  int64_t start = 0;
+  uint64_t deg = 0;
+  if (tid < total) {
+    uint32_t j = 0;
+#pragma unroll
+    for (int k = 1; k < kSegments; ++k) j += sh.seg_pre[k] <= tid;
+    v = in_seg[(size_t)j * a.n + (tid - sh.seg_pre[j])];
+    start = a.off[v];
+    deg = (uint64_t)(a.off[v + 1] - start);
+  }
+  // block-wide exclusive prefix of the degrees (64-bit)
+  uint64_t incl = deg;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t y = __shfl_up(incl, d);
+    if (lane >= (uint32_t)d) incl += y;
+  }
+  if (lane == kWave - 1) sh.u.f.wave_tot[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, T = 0;
+  for (int i = 0; i < W; ++i) {
+    const uint64_t t = sh.u.f.wave_tot[i];
+    if ((uint32_t)i < wave) before += t;
+    T += t;
+  }
+  sh.u.f.v[tid] = v;
+  sh.u.f.start[tid] = start;
+  sh.u.f.pre[tid] = before + incl - deg;
+  __syncthreads();
+  const uint64_t lo = T * gwave / nwaves, hi = T * (gwave + 1) / nwaves;
+  const uint32_t cnt = total ? total : 1;
+  for (uint64_t e0 = lo; e0 < hi; e0 += (uint64_t)kWave * kUnroll) {
+    bool act[kUnroll];
+    uint32_t u[kUnroll], w[kUnroll];
+#pragma unroll
+    for (int i = 0; i < kUnroll; ++i) {
+      const uint64_t e = e0 + (uint64_t)i * kWave + lane;
+      act[i] = e < hi;
+      // last entry whose prefix is <= e: entries of degree 0 share their
+      // prefix with the next one and are never chosen
+      uint32_t j = 0;
+      for (uint32_t step = 1u << 10; step >= 1; step >>= 1)
+        if (j + step < cnt && sh.u.f.pre[j + step] <= e) j += step;
+      u[i] = sh.u.f.v[j];
+      w[i] = act[i] ? a.col[sh.u.f.start[j] + (int64_t)(e - sh.u.f.pre[j])] : 0u;
+      if (act[i]) ++edges;
+    }
+    vis.visit_batch<kUnroll>(act, u, w);
+  }
+  __syncthreads();
+}
+
 // Phase H: the level's work items, round-robin over every wavefront of the
 // grid.  `total` items located through sh.item_pre.
 template <int BLOCK>
-__device__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t total,
+__device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t total,
                                 uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
   const uint32_t lane = lane_id();
   for (uint32_t i = gwave; i < total; i += nwaves) {
@@ -487,7 +571,7 @@ __device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t
 // collected in LDS; the wave then writes the chunk's visited words and the
 // next level's frontier-bitmap words itself (it owns them: no atomics).
 template <int BLOCK>
-__device__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t L,
+__device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t L,
                                const uint32_t *fb_cur, uint32_t *fb_nxt, uint32_t wave,
                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
   const uint32_t lane = lane_id();
@@ -595,7 +679,7 @@ __device__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &
 // arguments): initialisation, then the level loop until a level claims no
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
 template <int BLOCK>
-__device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
+__device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / kWave;
@@ -655,7 +739,9 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
   uint32_t bu_levels = 0;
   bool bottom_up = false, prev_bottom_up = false;
   for (uint32_t L = 0;; ++L) {
-    const uint32_t *in_seg = a.qseg[L & 1];
+    // selects, not a[L & 1]: an array member indexed at run time would put
+    // the whole argument block in scratch memory
+    const uint32_t *in_seg = (L & 1) ? a.qseg[1] : a.qseg[0];
     const LevelSlot &in_slot = ctl->slot[(L + 3) & 3];
     LevelSlot &cur = ctl->slot[L & 3];
     if (gtid == 0)
@@ -669,7 +755,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
       if (!bottom_up && n_cur > n_prev && m_f * a.alpha > m_u) bottom_up = true;
       else if (bottom_up && n_cur < a.bu_exit && n_cur < n_prev) bottom_up = false;
     }
-    vis.seg = a.qseg[(L + 1) & 1] + (size_t)sh.seg * a.n;
+    vis.seg = ((L & 1) ? a.qseg[0] : a.qseg[1]) + (size_t)sh.seg * a.n;
     vis.seg_tail = &cur.tail[sh.seg];
     vis.next_level = L + 1;
     vis.claims = 0;
@@ -682,7 +768,10 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
       // refresh the snapshot: words copied while claims go on are still
       // subsets of the visited set, which is all the filter needs
       for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
-      topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, trank, tsize, edges);
+      if (n_cur <= (uint64_t)BLOCK)
+        topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
+      else
+        topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, trank, tsize, edges);
       flush_block<BLOCK>(vis, sh, wave);
       if (a.prof && tid == 0) busy += now_ns() - t_start;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
@@ -773,7 +862,7 @@ __device__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
 // `xpt` XCDs each, whose barriers are team_bar(id).  Returns false in every
 // workgroup alike when a team would span more XCDs than frontier segments.
 template <int BLOCK, class TeamBar>
-__device__ bool kernel_prologue(GridBarrier *census_bar, uint32_t xpt, bool by_ticket, TeamBar team_bar,
+__device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_t xpt, bool by_ticket, TeamBar team_bar,
                                 SharedAll<BLOCK> &sh, Team &team) {
   if (threadIdx.x == 0) {
     CensusInfo ci;

@@ -73,11 +73,10 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 //   * workgroups arrive at their XCD's counter; the last one of each XCD
 //     writes that L2 back once (buffer_wbl2 sc1) and arrives at the root;
 //   * the root's last arrival publishes generation + payload sum in one
-//     64-bit word;
-//   * each XCD's leader invalidates its L2 (buffer_inv sc1) and then
-//     releases its XCD; every other workgroup invalidates again (device
-//     scope is the only one that drops its CU's L1 lines) before reading
-//     shared data.
+//     64-bit word, which every workgroup polls with relaxed loads;
+//   * on release every workgroup invalidates (buffer_inv sc1: device scope
+//     is the only one that drops its CU's L1 lines, and it also drops the
+//     XCD's L2 lines) before reading shared data.
 // Workgroups find their XCD from the XCC_ID hardware register; the number
 // of workgroups per XCD is counted at kernel start (barrier_census), because
 // the dispatcher's block -> XCD rotation changes from launch to launch.
@@ -94,7 +93,6 @@ struct alignas(256) BarrierWord {
 
 struct GridBarrier {
   BarrierWord xleaf[kMaxXcd];   // per-XCD arrivals
-  BarrierWord xrel[kMaxXcd];    // per-XCD release generation (low 32 bits)
   BarrierWord xside[kMaxXcd];   // per-XCD sum of the 64-bit side payload
   BarrierWord root;             // XCD-leader arrivals
   BarrierWord root_side;        // side payload sum over XCDs
@@ -237,7 +235,8 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
     }
     const uint64_t old = __hip_atomic_fetch_add(&b->xleaf[st.xcc].word, kArrive | payload,
                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint64_t total;
+    uint64_t total = 0;
+    bool released = false;
     if ((old >> 40) + 1 == st.xcc_n) {
       // XCD leader: every workgroup of this XCD has its stores in this L2
       const uint64_t xsum = (old & kPayloadMask) + payload;
@@ -259,22 +258,19 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
         wait_own_stores();
         __hip_atomic_store(&b->gensum.word, ((uint64_t)next << 40) | total, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        uint64_t w;
-        while (((w = ld_relaxed(&b->gensum.word)) >> 40) != next) __builtin_amdgcn_s_sleep(1);
-        total = w & kPayloadMask;
+        released = true;
       }
-      l2_invalidate();
-      __hip_atomic_store((uint32_t *)&b->xrel[st.xcc].word, next, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      while (ld_relaxed((uint32_t *)&b->xrel[st.xcc].word) != next) __builtin_amdgcn_s_sleep(1);
-      total = ld_relaxed(&b->gensum.word) & kPayloadMask;
-      // this CU's L1 may hold stale lines; on gfx950 only the device-scope
-      // invalidate reaches it (workgroup scope leaves L1 alone outside
-      // thread-group-split mode), and it also drops this XCD's L2 lines
-      l2_invalidate();
     }
+    if (!released) {
+      // every workgroup waits on the root's word itself (no per-XCD relay)
+      uint64_t w;
+      while (((w = ld_relaxed(&b->gensum.word)) >> 40) != next) __builtin_amdgcn_s_sleep(1);
+      total = w & kPayloadMask;
+    }
+    // this CU's L1 may hold stale lines; on gfx950 only the device-scope
+    // invalidate reaches it (workgroup scope leaves L1 alone outside
+    // thread-group-split mode), and it also drops this XCD's L2 lines
+    l2_invalidate();
     st.gen = next;
     s_word[0] = total;
     if (side_total) s_word[1] = ld_relaxed(&b->side_out.word);
@@ -325,6 +321,7 @@ struct Visitor {
   const int64_t *off;        // row offsets, read for claimed vertices when track_mf
   bool track_mf;
   uint64_t mf;               // out-degree sum of vertices claimed by this lane
+  uint32_t *fb_mark;         // when set, claims also set their bit here (next frontier bitmap)
 
   // Moves this wave's staged entries to the segment (one atomic).
   __device__ __forceinline__ void flush_wave() {
@@ -386,6 +383,9 @@ struct Visitor {
           level[w[i]] = next_level;
           if (parent) parent[w[i]] = u[i];
           if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+          if (fb_mark)
+            __hip_atomic_fetch_or(&fb_mark[w[i] >> 5], 1u << (w[i] & 31u), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
         }
       }
       const uint64_t mask = __ballot(claimed);
@@ -404,6 +404,8 @@ struct Visitor {
         level[w] = next_level;
         if (parent) parent[w] = u;
         if (track_mf) mf += (uint64_t)(off[w + 1] - off[w]);
+        if (fb_mark)
+          __hip_atomic_fetch_or(&fb_mark[w >> 5], 1u << (w & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     const uint64_t mask = __ballot(claimed);

@@ -159,7 +159,6 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.bottom_up = (g->cfg.direction_opt && g->symmetric) ? 1u : 0u;
   a.alpha = g->cfg.bu_enter_div;
   a.bu_exit = g->cfg.bu_exit_div ? g->n / g->cfg.bu_exit_div : 0;
-  a.count_edges = 0;
   a.m = g->m;
   return a;
 }

@@ -106,7 +106,6 @@ struct BfsArgs {
   uint32_t bottom_up;    // direction optimisation allowed (symmetric graph)
   uint32_t alpha;        // bottom-up when frontier edges > unexplored edges / alpha
   uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer vertices
-  uint32_t count_edges;  // sum the out-degrees of reached vertices into ctl->edges
   uint64_t m;            // CSR entries
 };
 
@@ -433,6 +432,10 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
   expand_large<BLOCK>(vis, sh, bn);
 }
 
+// Frontier edges from which a frontier of at most one entry per thread is
+// expanded edge-balanced (below, degree bins cost less).
+constexpr uint64_t kSmallFrontierEdges = 8192;
+
 // Top-down level of a frontier with at most BLOCK entries: every workgroup
 // reads the whole frontier and its degrees, and the team's waves split the
 // concatenated edge list evenly (edge-balanced, no bins, no work items, so
@@ -702,6 +705,9 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     a.vis[w] = init;
     a.vis_snap[w] = init;
   }
+  const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
+  if (a.bottom_up)
+    for (size_t w = gtid; w < 2 * fbs; w += gstride) a.fb[w] = 0;
   // level-0 frontier: the source alone, in segment 0 of buffer 0, whose
   // tails live in the slot of "level -1" (slot 3)
   if (gtid == 0) {
@@ -728,7 +734,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   vis.atomics = 0;
 
   vis.off = a.off;
-  vis.track_mf = a.bottom_up != 0 || a.count_edges != 0;
+  vis.fb_mark = nullptr;
+  // out-degree sums of the claimed vertices: the direction heuristic, the
+  // choice of the small-frontier expansion, and ctl->edges
+  vis.track_mf = true;
   vis.mf = 0;
 
   uint64_t n_cur = 1, n_prev = 0, reached = 1;
@@ -737,7 +746,15 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   uint64_t m_f = (uint64_t)(a.off[a.source + 1] - a.off[a.source]);
   uint64_t m_seen = m_f;
   uint32_t bu_levels = 0;
-  bool bottom_up = false, prev_bottom_up = false;
+  bool bottom_up = false;
+  // Frontier bitmaps alternate by level parity: fb[L & 1] is level L's.
+  // fb_zero bit b: fb[b] is known to be all zero; fb_valid: fb[L & 1] holds
+  // exactly level L's frontier.  A top-down level zeroes fb[L & 1] (unused
+  // there) and, when fb[(L + 1) & 1] is known zero, ORs its claims into it,
+  // so a bottom-up level right after it needs no level-array scan; a
+  // bottom-up level writes the next bitmap whole.
+  uint32_t fb_zero = 3;
+  bool fb_valid = false;
   for (uint32_t L = 0;; ++L) {
     // selects, not a[L & 1]: an array member indexed at run time would put
     // the whole argument block in scratch memory
@@ -768,7 +785,18 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       // refresh the snapshot: words copied while claims go on are still
       // subsets of the visited set, which is all the filter needs
       for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
-      if (n_cur <= (uint64_t)BLOCK)
+      bool fb_track = false;
+      if (a.bottom_up) {
+        uint32_t *fb_here = a.fb + (L & 1) * fbs;
+        for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
+        fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
+        vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
+        fb_zero = 1u << (L & 1);
+      }
+      fb_valid = fb_track;
+      // a short frontier with many edges (hubs): edge-balanced over the
+      // team in one pass; otherwise degree bins
+      if (n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges)
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
       else
         topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, trank, tsize, edges);
@@ -802,16 +830,15 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       }
     } else {
       ++bu_levels;
-      // frontier bitmaps alternate by level parity; a bottom-up level writes
-      // the next level's, so only the first after a top-down level scans
-      // the level array
-      const size_t fbs = (size_t)a.n_words + 1;
       uint32_t *fb_cur = a.fb + (L & 1) * fbs;
       uint32_t *fb_nxt = a.fb + ((L + 1) & 1) * fbs;
-      if (!prev_bottom_up) {
+      if (!fb_valid) {
         build_frontier_bitmap(a, L, fb_cur, gwave, nwaves);
         grid_barrier(&ctl->bar, bst, 0, sh.bcast);
       }
+      vis.fb_mark = nullptr;
+      fb_zero = 0;
+      fb_valid = true;
       uint64_t tb = 0;
       if (a.prof && tid == 0) tb = now_ns();
       bottomup_level<BLOCK>(a, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
@@ -823,7 +850,6 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       m_f = mf1;
     }
     m_seen += m_f;
-    prev_bottom_up = bottom_up;
     if (a.prof) {
       prof_add<BLOCK>(sh, kProfEdges, edges);
       prof_add<BLOCK>(sh, kProfProbes, vis.probes);
@@ -849,7 +875,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         ctl->depth = L + 1;
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
-        ctl->edges = vis.track_mf ? m_seen : 0;
+        ctl->edges = m_seen;
       }
       break;
     }

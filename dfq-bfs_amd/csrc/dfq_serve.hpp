@@ -232,7 +232,6 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.bottom_up = ((d.flags & kServeDirOpt) && g.symmetric) ? 1u : 0u;
     a.alpha = d.alpha;
     a.bu_exit = d.beta ? g.n / d.beta : 0;
-    a.count_edges = 1;
     a.m = g.m;
     grid_bfs<BLOCK>(a, sh, team);
     // outputs reach memory the host's copies read before the result appears

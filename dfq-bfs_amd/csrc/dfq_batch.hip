@@ -106,7 +106,7 @@ dfq_query_plan default_plan(uint32_t bucket) {
   p.medium_max = 256;
   p.large_max = 256;
   p.direction_opt = 1;
-  p.alpha = 14;
+  p.alpha = 40;
   p.beta = 24;
   return p;
 }

@@ -205,7 +205,7 @@ int dfq_config_default(dfq_config *cfg) {
   cfg->medium_max = 256;
   cfg->large_max = 256;
   cfg->direction_opt = 0;
-  cfg->bu_enter_div = 14;
+  cfg->bu_enter_div = 40;
   cfg->bu_exit_div = 24;
   return DFQ_OK;
 }

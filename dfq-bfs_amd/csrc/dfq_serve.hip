@@ -163,7 +163,7 @@ dfq_query_plan default_serve_plan() {
   p.medium_max = 256;
   p.large_max = 256;
   p.direction_opt = 1;
-  p.alpha = 14;
+  p.alpha = 40;
   p.beta = 24;
   return p;
 }

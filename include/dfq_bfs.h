@@ -42,11 +42,13 @@ extern "C" {
  *              deg <= small_max       -> lane-owned, wave-gathered edges
  *              deg <= medium_max      -> one wavefront (64 lanes) per vertex
  *              deg <= large_max       -> one workgroup per vertex
- *              deg >  large_max       -> whole grid, HBM queue
+ *              deg >  large_max       -> 256-entry work items over the grid
  * direction_opt = 1 lets a level run bottom-up (every unvisited vertex looks
  * for a parent in the frontier) when the frontier is growing and its
- * out-edges exceed the unexplored edges / bu_enter_div (Beamer's alpha,
- * default 14), and returns to top-down when a shrinking frontier holds fewer
+ * out-edges exceed the unexplored edges / bu_enter_div (Beamer's alpha;
+ * default 40, measured on MI355X, where a top-down level costs relatively
+ * more than on the CPUs alpha = 14 was tuned for), and returns to top-down
+ * when a shrinking frontier holds fewer
  * than n / bu_exit_div vertices (beta, default 24).  It only takes effect on
  * symmetric graphs (checked when the graph is created); levels are the same
  * either way.  direction_opt = 0 runs every level top-down. */

@@ -12,7 +12,9 @@ launch per source, outputs written to device buffers that stay resident).
 `value` is the harmonic mean over the sources of the per-source GTEPS,
 GTEPS = (E_r / 2) / t_kernel with E_r the directed CSR entries of the
 reached component (Graph500 convention for undirected graphs) and t_kernel
-the hipEvent time of the launch on its stream, averaged over the timed
+the hipEvent time of the launch on its stream (events recorded right before
+and after each launch, with all launches of a step queued back to back so
+the host's submission latency is not counted), averaged over the timed
 steps.  The roofline entry prices the algorithmic bytes of one search,
 B_alg = 4 E_r + 16 V_r + 4 N (CSR entries, two int64 offsets per reached
 vertex, the level array written in full), against 8 TB/s.
@@ -54,14 +56,14 @@ MODES = {
 def run_mode(dfq, g, srcs, work, steps, warmup, d_level):
     """Per-source mean kernel ms over the timed steps, and wall ms per step."""
     for _ in range(warmup):
-        for s in srcs:
-            g.bfs_device(int(s), d_level.ptr)
+        g.bfs_device_many(srcs, d_level.ptr)
     dfq.synchronize()
     times = np.zeros((steps, len(srcs)))
     t0 = time.perf_counter()
     for k in range(steps):
-        for i, s in enumerate(srcs):
-            times[k, i] = g.bfs_device(int(s), d_level.ptr).kernel_ms
+        # one step: a search from every source, queued back to back, each
+        # bracketed by hipEvents on the engine's stream
+        times[k] = [st.kernel_ms for st in g.bfs_device_many(srcs, d_level.ptr)]
     dfq.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
     t = times.mean(axis=0)

@@ -207,6 +207,7 @@ int build_tables(dfq_batch *b, const std::vector<uint32_t> &subset) {
   s.ctl = b->d_ctl;
   s.prof = nullptr;
   s.prof_levels = 0;
+  s.result = nullptr;
   return DFQ_OK;
 }
 

@@ -70,6 +70,10 @@ struct dfq_graph {
   uint64_t *d_prof = nullptr;
   uint32_t prof_levels = 0;
   Control *d_ctl = nullptr;
+  // back-to-back searches: events between launches, per-launch results
+  std::vector<hipEvent_t> evs;
+  uint64_t *d_res = nullptr;
+  uint32_t res_cap = 0;
   dfq_config cfg{};
   int grid = 0;
   int block = 0;
@@ -127,6 +131,8 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_items);
   (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
+  (void)hipFree(g->d_res);
+  for (hipEvent_t e : g->evs) (void)hipEventDestroy(e);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -143,6 +149,7 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.fb = g->d_fb;
   a.iso = g->d_iso;
   a.col_bu = g->d_col_bu;
+  a.result = nullptr;
   a.qseg[0] = g->d_qseg[0];
   a.qseg[1] = g->d_qseg[1];
   a.items = g->d_items;
@@ -369,6 +376,56 @@ int dfq_bfs(dfq_graph *g, uint32_t source, uint32_t *level_out, uint32_t *parent
 int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats) {
   if (!g || !d_level) return DFQ_ERR_INVALID_ARG;
   return run_bfs(g, source, d_level, d_parent, stats);
+}
+
+int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level,
+                        uint32_t *d_parent, dfq_stats *stats) {
+  if (!g || !d_level || (count && (!sources || !stats))) return DFQ_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < count; ++i)
+    if (sources[i] >= g->n) return DFQ_ERR_RANGE;
+  if (count == 0) return DFQ_OK;
+  HIP_TRY(hipSetDevice(g->device));
+  while (g->evs.size() < (size_t)count + 1) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    g->evs.push_back(e);
+  }
+  if (count > g->res_cap) {
+    (void)hipFree(g->d_res);
+    g->d_res = nullptr;
+    g->res_cap = 0;
+    HIP_TRY(hipMalloc(&g->d_res, sizeof(uint64_t) * 4 * count));
+    g->res_cap = count;
+  }
+  // all launches are queued before the host waits, so each event pair
+  // brackets one kernel and not the host's submission of it
+  HIP_TRY(hipEventRecord(g->evs[0], g->stream));
+  for (uint32_t i = 0; i < count; ++i) {
+    BfsArgs a = make_args(g, sources[i], d_level, d_parent, false);
+    a.result = g->d_res + 4 * (size_t)i;
+    void *args[] = {&a};
+    HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0,
+                                       g->stream));
+    HIP_TRY(hipEventRecord(g->evs[i + 1], g->stream));
+  }
+  HIP_TRY(hipEventSynchronize(g->evs[count]));
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpy(&err, &g->d_ctl->err, sizeof(err), hipMemcpyDeviceToHost));
+  if (err) return DFQ_ERR_CAPACITY;
+  std::vector<uint64_t> res(4 * (size_t)count);
+  HIP_TRY(hipMemcpy(res.data(), g->d_res, sizeof(uint64_t) * res.size(), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < count; ++i) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g->evs[i], g->evs[i + 1]));
+    dfq_stats &st = stats[i];
+    st.kernel_ms = ms;
+    st.depth = (uint32_t)res[4 * i];
+    st.reached = res[4 * i + 1];
+    st.bottom_up_levels = (uint32_t)res[4 * i + 2];
+    st.grid_blocks = (uint32_t)g->grid;
+    st.block_size = (uint32_t)g->block;
+  }
+  return DFQ_OK;
 }
 
 int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows, uint32_t max_levels, dfq_stats *stats) {

@@ -96,6 +96,7 @@ struct BfsArgs {
   uint32_t item_cap;     // items per segment (m / kItem + n bounds one level)
   Control *ctl;
   uint64_t *prof;        // [prof_levels][kProfFields] or null
+  uint64_t *result;      // [4]: depth, reached, bottom-up levels, edges; or null
   uint32_t prof_levels;
   uint32_t n;
   uint32_t n_words;
@@ -876,6 +877,12 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
         ctl->edges = m_seen;
+        if (a.result) {
+          a.result[0] = L + 1;
+          a.result[1] = reached;
+          a.result[2] = bu_levels;
+          a.result[3] = m_seen;
+        }
       }
       break;
     }

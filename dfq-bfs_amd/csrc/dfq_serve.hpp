@@ -223,6 +223,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.ctl = &c->team[id];
     a.prof = nullptr;
     a.prof_levels = 0;
+    a.result = nullptr;
     a.n = g.n;
     a.n_words = (g.n + 31) / 32;
     a.source = d.source;

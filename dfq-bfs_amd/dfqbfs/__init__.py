@@ -141,6 +141,7 @@ _sig("dfq_graph_get_config", C.c_int, _P, C.POINTER(Config))
 _sig("dfq_graph_device_arrays", C.c_int, _P, C.POINTER(_P), C.POINTER(_P))
 _sig("dfq_bfs", C.c_int, _P, C.c_uint32, _u32p, _u32p, C.POINTER(Stats))
 _sig("dfq_bfs_device", C.c_int, _P, C.c_uint32, _P, _P, C.POINTER(Stats))
+_sig("dfq_bfs_device_many", C.c_int, _P, C.c_uint32, _u32p, _P, _P, C.POINTER(Stats))
 _sig("dfq_bfs_profile", C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32,
      C.POINTER(Stats))
 _sig("dfq_graph_is_symmetric", C.c_int, _P)
@@ -379,6 +380,15 @@ class DeviceGraph:
                                   max_levels, C.byref(st)), "dfq_bfs_profile")
         depth = min(st.depth, max_levels)
         return [dict(zip(self.PROFILE_FIELDS, map(int, rows[i]))) for i in range(depth)], st
+
+    def bfs_device_many(self, sources, d_level: int, d_parent: int | None = None):
+        """Searches from every source back to back (outputs overwritten in
+        turn); per-search Stats with event-bracketed kernel times."""
+        so = np.ascontiguousarray(sources, np.uint32)
+        st = (Stats * max(so.size, 1))()
+        check(lib.dfq_bfs_device_many(self._h, so.size, _u32(so), d_level, d_parent, st),
+              "dfq_bfs_device_many")
+        return list(st[:so.size])
 
     def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
         """BFS into caller-owned device buffers (raw device addresses)."""

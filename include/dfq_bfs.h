@@ -125,6 +125,15 @@ int dfq_graph_is_symmetric(const dfq_graph *g);
 int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
                    uint32_t *d_parent, dfq_stats *stats);
 
+/* count searches back to back on the graph's stream, outputs in device
+ * memory (each search overwrites the previous one's), one wait at the end.
+ * stats[i] belongs to sources[i]; its kernel_ms is bracketed by hipEvents
+ * recorded on the stream right before and after that launch, with every
+ * launch already queued, so it does not include the host's submission
+ * latency that a single dfq_bfs_device call does. */
+int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources,
+                        uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats);
+
 /* BFS with per-level counters (the reference's frontier profiling build,
  * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,140-144,
  * 200-209).  rows[max_levels][DFQ_PROFILE_FIELDS], one row per level:

@@ -218,3 +218,25 @@ def test_device_outputs(dfq, oracle):
         assert csr.validate(s, level, parent)["errors"] == 0
         st = g.bfs_device(s, lv.ptr, None)  # parents optional
         np.testing.assert_array_equal(lv.to_numpy(), level)
+
+
+@pytest.mark.parametrize("direction_opt", [0, 1])
+def test_device_many_back_to_back(dfq, oracle, direction_opt):
+    """dfq_bfs_device_many: per-search results equal single searches'; the
+    output buffer holds the last source's levels."""
+    csr = dfq.Csr.rmat(14, 16, seed=77)
+    srcs = [int(s) for s in csr.pick_sources(12, 5)]
+    lv = dfq.DeviceBuffer(4 * csr.n)
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(direction_opt=direction_opt)
+        say("device many", direction_opt)
+        stats = g.bfs_device_many(srcs, lv.ptr)
+        assert len(stats) == len(srcs)
+        for s, st in zip(srcs, stats):
+            e = oracle.bfs(csr.row_offsets, csr.col_idx, s)
+            assert st.kernel_ms > 0
+            assert st.reached == int((e != UNREACHED).sum())
+            assert st.depth == int(e[e != UNREACHED].max()) + 1
+        np.testing.assert_array_equal(lv.to_numpy(), oracle.bfs(csr.row_offsets, csr.col_idx, srcs[-1]))
+        with pytest.raises(dfq.DfqError):
+            g.bfs_device_many([csr.n], lv.ptr)

@@ -61,6 +61,8 @@ struct Control {
   uint32_t pad0;
   uint64_t edges;     // CSR entries of reached vertices (when counted)
   uint32_t pad[56];
+  BarrierWord solo_rel;        // tag of the last hand-back of solo levels
+  uint64_t solo_state[32];     // search state handed back (first 8 words used)
 };
 
 // Profiling counters, one row per level (optional).
@@ -679,6 +681,66 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
   }
 }
 
+// Solo levels: at most this many frontier vertices and out-edges.
+constexpr uint64_t kSoloMaxFrontier = 1024;
+constexpr uint64_t kSoloMaxEdges = 8192;
+
+// End of a solo level: the workgroup's claims and out-degree sum, and its
+// stores made visible to all its waves (same CU, so no cache maintenance).
+template <int BLOCK>
+__device__ __forceinline__ uint64_t solo_sync(SharedAll<BLOCK> &sh, uint32_t claims, uint64_t mf,
+                                              uint64_t *mf_out) {
+  wait_own_stores();
+  if (threadIdx.x == 0) sh.bcast[0] = claims;
+  __syncthreads();
+  *mf_out = mf;
+  const uint64_t n = sh.bcast[0];
+  __syncthreads();
+  return n;
+}
+
+// The solo workgroup hands the search back (or reports its end): its XCD's
+// L2 is written back, the state goes to the control block, then the tag.
+__device__ __forceinline__ void solo_publish(Control *ctl, uint64_t tag, uint32_t L, uint64_t n_cur,
+                                             uint64_t n_prev, uint64_t m_f, uint64_t m_seen, uint64_t reached,
+                                             uint32_t bu_levels, bool done) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    l2_writeback();  // waits for this thread's stores; the others waited before the barrier
+    const uint64_t st[8] = {L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, done ? 1ull : 0ull};
+    for (int i = 0; i < 8; ++i)
+      __hip_atomic_store(&ctl->solo_state[i], st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_own_stores();
+    __hip_atomic_store(&ctl->solo_rel.word, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
+// Other workgroups: wait for the solo workgroup's tag, then take the state.
+// Returns true when the search ended during the solo levels.
+template <int BLOCK>
+__device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, uint64_t tag, uint32_t &L,
+                                          uint64_t &n_cur, uint64_t &n_prev, uint64_t &m_f, uint64_t &m_seen,
+                                          uint64_t &reached, uint32_t &bu_levels) {
+  __shared__ uint64_t st[8];
+  if (threadIdx.x == 0) {
+    while (ld_relaxed(&ctl->solo_rel.word) != tag) __builtin_amdgcn_s_sleep(2);
+    l2_invalidate();
+    for (int i = 0; i < 8; ++i) st[i] = ld_relaxed(&ctl->solo_state[i]);
+  }
+  __syncthreads();
+  L = (uint32_t)st[0];
+  n_cur = st[1];
+  n_prev = st[2];
+  m_f = st[3];
+  m_seen = st[4];
+  reached = st[5];
+  bu_levels = (uint32_t)st[6];
+  const bool done = st[7] != 0;
+  __syncthreads();
+  return done;
+}
+
 // One BFS over the whole grid (every workgroup must call it with the same
 // arguments): initialisation, then the level loop until a level claims no
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
@@ -695,6 +757,9 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   BarrierState &bst = team.bst;
   Control *ctl = a.ctl;
   if (tid < kProfFields) sh.prof[tid] = 0;
+
+  uint64_t solo_next = 0;  // tag of the next hand-back of solo levels (thread 0)
+  if (tid == 0) solo_next = ld_relaxed(&ctl->solo_rel.word) + 1;
 
   // ---- initialisation (reference :216 cost init, done on the device) ----
   for (uint32_t i = gtid; i < a.n; i += gstride) {
@@ -756,6 +821,12 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   // bottom-up level writes the next bitmap whole.
   uint32_t fb_zero = 3;
   bool fb_valid = false;
+  // Solo levels: a small top-down level (and the small ones after it) is
+  // run by the team's first workgroup alone, separated by workgroup
+  // barriers only, while the others wait for it to publish the search
+  // state (or the end of the search).
+  bool solo = false;
+  uint64_t solo_tag = 0;
   for (uint32_t L = 0;; ++L) {
     // selects, not a[L & 1]: an array member indexed at run time would put
     // the whole argument block in scratch memory
@@ -773,6 +844,28 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       if (!bottom_up && n_cur > n_prev && m_f * a.alpha > m_u) bottom_up = true;
       else if (bottom_up && n_cur < a.bu_exit && n_cur < n_prev) bottom_up = false;
     }
+    const bool small = !bottom_up && tsize > 1 && n_cur <= kSoloMaxFrontier && m_f <= kSoloMaxEdges;
+    if (solo && !small) {
+      solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, false);
+      solo = false;
+    } else if (!solo && small) {
+      // tags count hand-backs: every workgroup read the last one before
+      // the search's first barrier
+      if (tid == 0) solo_tag = solo_next++;
+      if (trank != 0) {
+        // wait until the first workgroup is done with the small levels
+        const bool done = solo_wait(ctl, sh, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels);
+        if (done) break;
+        // solo levels leave the frontier bitmaps alone (fb_zero holds) but
+        // do not build the next one
+        fb_valid = false;
+        --L;  // the loop's ++L gives the level to resume at
+        continue;
+      }
+      solo = true;
+    }
+    // this workgroup's share: the whole team's, or alone in solo levels
+    const uint32_t r_gwave = solo ? wave : gwave, r_nwaves = solo ? (uint32_t)S::kWaves : nwaves;
     vis.seg = ((L & 1) ? a.qseg[0] : a.qseg[1]) + (size_t)sh.seg * a.n;
     vis.seg_tail = &cur.tail[sh.seg];
     vis.next_level = L + 1;
@@ -783,29 +876,40 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     uint64_t t_start = 0, busy = 0;
     if (a.prof && tid == 0) t_start = now_ns();
     if (!bottom_up) {
-      // refresh the snapshot: words copied while claims go on are still
-      // subsets of the visited set, which is all the filter needs
-      for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
       bool fb_track = false;
-      if (a.bottom_up) {
-        uint32_t *fb_here = a.fb + (L & 1) * fbs;
-        for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
-        fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
-        vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
-        fb_zero = 1u << (L & 1);
+      if (solo) {
+        // the snapshot of the last team level stays a valid (older) filter;
+        // the frontier bitmaps are not touched
+        vis.fb_mark = nullptr;
+      } else {
+        // refresh the snapshot: words copied while claims go on are still
+        // subsets of the visited set, which is all the filter needs
+        for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
+        if (a.bottom_up) {
+          uint32_t *fb_here = a.fb + (L & 1) * fbs;
+          for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
+          fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
+          vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
+          fb_zero = 1u << (L & 1);
+        }
       }
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the
       // team in one pass; otherwise degree bins
-      if (n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges)
+      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges)
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
       else
-        topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, trank, tsize, edges);
+        topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
+                               edges);
       flush_block<BLOCK>(vis, sh, wave);
       if (a.prof && tid == 0) busy += now_ns() - t_start;
       const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
       uint64_t mf1 = 0;
-      n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      if (solo) {
+        n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      } else {
+        n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      }
       m_f = mf1;
       if (tid == 0) {
         uint32_t t = 0;
@@ -821,12 +925,15 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         const uint32_t before = vis.claims;
         uint64_t th = 0;
         if (a.prof && tid == 0) th = now_ns();
-        topdown_phase_h<BLOCK>(a, sh, vis, hc, gwave, nwaves, edges);
+        topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         flush_block<BLOCK>(vis, sh, wave);
         if (a.prof && tid == 0) busy += now_ns() - th;
         const uint32_t wg2 = block_sum_uniform<BLOCK>(sh, vis.claims - before, wave);
         uint64_t mf2 = 0;
-        n_next += grid_barrier(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
+        if (solo)
+          n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
+        else
+          n_next += grid_barrier(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
         m_f += mf2;
       }
     } else {
@@ -884,6 +991,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
           a.result[3] = m_seen;
         }
       }
+      if (solo) solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, true);
       break;
     }
     n_prev = n_cur;

@@ -98,10 +98,9 @@ struct GridBarrier {
   BarrierWord root_side;        // side payload sum over XCDs
   BarrierWord side_out;         // side payload total of the last round
   BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
-  BarrierWord census_total;     // workgroups counted at kernel start
-  BarrierWord census_done;      // workgroups that have read the census
+  BarrierWord census_epoch;     // launches counted so far (low bit picks the census set)
   BarrierWord ticket;           // workgroup ranks handed out within a team
-  uint32_t census[kMaxXcd];     // workgroups per XCD
+  uint32_t census[2][kMaxXcd];  // workgroups per XCD, one set per launch parity
 };
 
 // Barrier state of one workgroup within its team (thread 0's copy counts).
@@ -117,7 +116,6 @@ struct CensusInfo {
   uint32_t xcc;                // this workgroup's XCD
   uint32_t xrank;              // its rank among the XCDs that have workgroups
   uint32_t nx;                 // XCDs with workgroups
-  uint32_t count[kMaxXcd];     // workgroups per XCD rank
 };
 
 // A team: the workgroups of `xpt` consecutive XCD ranks, with its own
@@ -155,58 +153,69 @@ __device__ __forceinline__ T ld_relaxed(T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Kernel start (thread 0 of every workgroup, all workgroups resident):
-// counts workgroups per XCD.  The last workgroup to finish reading clears
-// the counters for the next launch.
-__device__ __forceinline__ void barrier_census(GridBarrier *b, CensusInfo &ci) {
+// Kernel start (wave 0 of every workgroup, all workgroups resident):
+// counts workgroups per XCD.  Alternate launches count in alternate sets
+// (census_epoch's low bit), and once a launch's count is complete its
+// workgroup 0 clears the other set for the next launch and advances the
+// epoch, so no workgroup waits for the last reader.  Lanes 0..15 poll the
+// sixteen counters with one load; lanes 16..31 read the barrier generation
+// of teams 0..15 alongside (stable until the team's first barrier, which
+// needs this workgroup).  Results: `count` holds the non-zero counters in
+// XCD order and `gen` the generations, both in LDS.
+template <class TeamBar>
+__device__ __forceinline__ void barrier_census(GridBarrier *b, TeamBar team_bar, CensusInfo &ci,
+                                               uint32_t *count, uint32_t *gen) {
+  const uint32_t lane = lane_id();
   ci.xcc = xcc_id();
-  // the per-XCD count must land before the total that releases the readers
-  __hip_atomic_fetch_add(&b->census[ci.xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  wait_own_stores();
-  __hip_atomic_fetch_add(&b->census_total.word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  while (ld_relaxed(&b->census_total.word) < gridDim.x) __builtin_amdgcn_s_sleep(1);
-  uint32_t nx = 0, xrank = 0;
-  for (int i = 0; i < kMaxXcd; ++i) ci.count[i] = 0;
-  for (int i = 0; i < kMaxXcd; ++i) {
-    const uint32_t c = ld_relaxed(&b->census[i]);
-    if (c == 0) continue;
-    if ((uint32_t)i < ci.xcc) ++xrank;
-    ci.count[nx++] = c;
+  const uint64_t epoch = ld_relaxed(&b->census_epoch.word);
+  uint32_t *cnt = b->census[epoch & 1];
+  if (lane == 0) __hip_atomic_fetch_add(&cnt[ci.xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t c = 0, g = 0;
+  for (;;) {
+    if (lane < (uint32_t)kMaxXcd) c = ld_relaxed(&cnt[lane]);
+    else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> 40);
+    uint32_t sum = c;
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    sum += __shfl_xor(sum, 4);
+    sum += __shfl_xor(sum, 8);
+    if ((uint32_t)__shfl(sum, 0) >= gridDim.x) break;
+    __builtin_amdgcn_s_sleep(1);
   }
-  ci.nx = nx;
-  ci.xrank = xrank;
-  wait_own_stores();
-  const uint64_t done = __hip_atomic_fetch_add(&b->census_done.word, 1ull, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-  if (done + 1 == gridDim.x) {
-    // everyone has read: reset for the next launch
-    for (int i = 0; i < kMaxXcd; ++i)
-      __hip_atomic_store(&b->census[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&b->census_total.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&b->census_done.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t nz = (uint32_t)__ballot(lane < (uint32_t)kMaxXcd && c != 0);
+  const uint32_t below = nz & ((1u << (lane & 31u)) - 1u);
+  if (lane < (uint32_t)kMaxXcd && c != 0) count[__popc(below)] = c;
+  if (lane >= (uint32_t)kMaxXcd && lane < 2u * kMaxXcd) gen[lane - kMaxXcd] = g;
+  ci.nx = __popc(nz);
+  ci.xrank = __popc(nz & ((1u << ci.xcc) - 1u));
+  if (blockIdx.x == 0 && lane == 0) {
+    uint32_t *other = b->census[(epoch & 1) ^ 1];
+    for (int i = 0; i < kMaxXcd; ++i) __hip_atomic_store(&other[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&b->census_epoch.word, epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
 // Team of XCD ranks [id * xpt, id * xpt + xpt) (thread 0).  `tb` is the
-// team's barrier.  by_ticket: ranks from the team's ticket counter (reset
-// by team_ticket_reset after the team's first barrier); otherwise the
-// grid is one team and the rank is blockIdx.x.
-__device__ __forceinline__ void team_form(const CensusInfo &ci, uint32_t xpt, GridBarrier *tb, bool by_ticket,
-                                          Team &t) {
+// team's barrier; `count` and `gen` are the census results.  by_ticket:
+// ranks from the team's ticket counter (reset by team_ticket_reset after
+// the team's first barrier); otherwise the grid is one team and the rank
+// is blockIdx.x.
+__device__ __forceinline__ void team_form(const CensusInfo &ci, const uint32_t *count, const uint32_t *gen,
+                                          uint32_t xpt, GridBarrier *tb, bool by_ticket, Team &t) {
   t.id = ci.xrank / xpt;
   const uint32_t first = t.id * xpt;
   const uint32_t last = min(first + xpt, ci.nx);
   uint32_t size = 0;
-  for (uint32_t r = first; r < last; ++r) size += ci.count[r];
+  for (uint32_t r = first; r < last; ++r) size += count[r];
   t.size = size;
   t.seg = ci.xrank - first;
   t.teams = (ci.nx + xpt - 1) / xpt;
   t.rank = by_ticket ? (uint32_t)__hip_atomic_fetch_add(&tb->ticket.word, 1ull, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT)
                      : blockIdx.x;
-  t.bst.gen = (uint32_t)(ld_relaxed(&tb->gensum.word) >> 40);
+  t.bst.gen = gen[t.id];
   t.bst.xcc = ci.xcc;
-  t.bst.xcc_n = ci.count[ci.xrank];
+  t.bst.xcc_n = count[ci.xrank];
   t.bst.nx = last - first;
 }
 

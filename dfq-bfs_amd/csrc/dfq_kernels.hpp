@@ -171,6 +171,7 @@ struct SharedAll {
   uint64_t bcast[2];
   uint64_t mf_sum;
   uint32_t med_n, big_n;
+  uint32_t census[2 * kMaxXcd];  // kernel start: workgroups per XCD rank, team generations
 };
 
 // Wave-uniform gather of the edges of up to 64 lane-owned vertices.
@@ -1005,11 +1006,12 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
 template <int BLOCK, class TeamBar>
 __device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_t xpt, bool by_ticket, TeamBar team_bar,
                                 SharedAll<BLOCK> &sh, Team &team) {
+  CensusInfo ci;
+  if (threadIdx.x < kWave) barrier_census(census_bar, team_bar, ci, sh.census, sh.census + kMaxXcd);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    CensusInfo ci;
-    barrier_census(census_bar, ci);
     if (xpt == 0 || xpt > ci.nx) xpt = ci.nx;
-    team_form(ci, xpt, team_bar(ci.xrank / xpt), by_ticket, team);
+    team_form(ci, sh.census, sh.census + kMaxXcd, xpt, team_bar(ci.xrank / xpt), by_ticket, team);
     sh.seg = team.seg;
     sh.team_rank = team.rank;
     sh.team_size = team.size;

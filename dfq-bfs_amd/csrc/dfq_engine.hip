@@ -70,12 +70,15 @@ struct dfq_graph {
   uint64_t *d_prof = nullptr;
   uint32_t prof_levels = 0;
   Control *d_ctl = nullptr;
-  // back-to-back searches: events between launches, per-launch results
+  // back-to-back searches: events between launches, per-search results,
+  // the sources of a one-launch sequence
   std::vector<hipEvent_t> evs;
   uint64_t *d_res = nullptr;
+  uint32_t *d_src = nullptr;
   uint32_t res_cap = 0;
   dfq_config cfg{};
   int grid = 0;
+  int grid_seq = 0;  // the sequence kernel's grid (its own occupancy)
   int block = 0;
 };
 
@@ -99,6 +102,14 @@ const void *kernel_for(uint32_t block) {
   }
 }
 
+const void *seq_kernel_for(uint32_t block) {
+  switch (block) {
+    case 1024: return (const void *)&dfq_bfs_seq_kernel<1024>;
+    case 512: return (const void *)&dfq_bfs_seq_kernel<512>;
+    default: return (const void *)&dfq_bfs_seq_kernel<256>;
+  }
+}
+
 // Grid = CUs x co-resident workgroups per CU (a cooperative launch needs
 // every workgroup resident).
 int configure(dfq_graph *g, const dfq_config &cfg) {
@@ -110,8 +121,14 @@ int configure(dfq_graph *g, const dfq_config &cfg) {
                                                         (int)cfg.block_size, 0));
   if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
   if (cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)cfg.blocks_per_cu);
+  int per_cu_seq = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_seq, seq_kernel_for(cfg.block_size),
+                                                        (int)cfg.block_size, 0));
+  if (per_cu_seq <= 0) return DFQ_ERR_COOPERATIVE;
+  per_cu_seq = std::min(per_cu_seq, per_cu);
   g->cfg = cfg;
   g->grid = prop.multiProcessorCount * per_cu;
+  g->grid_seq = prop.multiProcessorCount * per_cu_seq;
   g->block = (int)cfg.block_size;
   return DFQ_OK;
 }
@@ -132,6 +149,7 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
   (void)hipFree(g->d_res);
+  (void)hipFree(g->d_src);
   for (hipEvent_t e : g->evs) (void)hipEventDestroy(e);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
@@ -168,6 +186,31 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.bu_exit = g->cfg.bu_exit_div ? g->n / g->cfg.bu_exit_div : 0;
   a.m = g->m;
   return a;
+}
+
+// Room for `count` searches' results and sources.
+hipError_t reserve_results(dfq_graph *g, uint32_t count) {
+  if (count <= g->res_cap) return hipSuccess;
+  (void)hipFree(g->d_res);
+  (void)hipFree(g->d_src);
+  g->d_res = nullptr;
+  g->d_src = nullptr;
+  g->res_cap = 0;
+  hipError_t e = hipMalloc(&g->d_res, sizeof(uint64_t) * kResultWords * count);
+  if (e == hipSuccess) e = hipMalloc(&g->d_src, sizeof(uint32_t) * count);
+  if (e != hipSuccess) return e;
+  g->res_cap = count;
+  return hipSuccess;
+}
+
+// Stats of one search from its kResultWords result words.
+void fill_stats(const dfq_graph *g, const uint64_t *r, double ms, int grid, dfq_stats &st) {
+  st.kernel_ms = ms;
+  st.depth = (uint32_t)r[0];
+  st.reached = r[1];
+  st.bottom_up_levels = (uint32_t)r[2];
+  st.grid_blocks = (uint32_t)grid;
+  st.block_size = (uint32_t)g->block;
 }
 
 int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats,
@@ -390,19 +433,13 @@ int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources, u
     HIP_TRY(hipEventCreate(&e));
     g->evs.push_back(e);
   }
-  if (count > g->res_cap) {
-    (void)hipFree(g->d_res);
-    g->d_res = nullptr;
-    g->res_cap = 0;
-    HIP_TRY(hipMalloc(&g->d_res, sizeof(uint64_t) * 4 * count));
-    g->res_cap = count;
-  }
+  HIP_TRY(reserve_results(g, count));
   // all launches are queued before the host waits, so each event pair
   // brackets one kernel and not the host's submission of it
   HIP_TRY(hipEventRecord(g->evs[0], g->stream));
   for (uint32_t i = 0; i < count; ++i) {
     BfsArgs a = make_args(g, sources[i], d_level, d_parent, false);
-    a.result = g->d_res + 4 * (size_t)i;
+    a.result = g->d_res + (size_t)kResultWords * i;
     void *args[] = {&a};
     HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0,
                                        g->stream));
@@ -412,18 +449,54 @@ int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources, u
   uint32_t err = 0;
   HIP_TRY(hipMemcpy(&err, &g->d_ctl->err, sizeof(err), hipMemcpyDeviceToHost));
   if (err) return DFQ_ERR_CAPACITY;
-  std::vector<uint64_t> res(4 * (size_t)count);
+  std::vector<uint64_t> res((size_t)kResultWords * count);
   HIP_TRY(hipMemcpy(res.data(), g->d_res, sizeof(uint64_t) * res.size(), hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < count; ++i) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, g->evs[i], g->evs[i + 1]));
-    dfq_stats &st = stats[i];
-    st.kernel_ms = ms;
-    st.depth = (uint32_t)res[4 * i];
-    st.reached = res[4 * i + 1];
-    st.bottom_up_levels = (uint32_t)res[4 * i + 2];
-    st.grid_blocks = (uint32_t)g->grid;
-    st.block_size = (uint32_t)g->block;
+    fill_stats(g, &res[(size_t)kResultWords * i], ms, g->grid, stats[i]);
+  }
+  return DFQ_OK;
+}
+
+int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level,
+                       uint64_t level_stride, uint32_t *d_parent, uint64_t parent_stride, dfq_stats *stats,
+                       double *launch_ms) {
+  if (!g || !d_level || (count && (!sources || !stats))) return DFQ_ERR_INVALID_ARG;
+  if ((level_stride && level_stride < g->n) || (d_parent && parent_stride && parent_stride < g->n))
+    return DFQ_ERR_INVALID_ARG;
+  // element offsets of the last search must fit the kernel's 64-bit index
+  if (count && (level_stride > UINT64_MAX / count || parent_stride > UINT64_MAX / count))
+    return DFQ_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < count; ++i)
+    if (sources[i] >= g->n) return DFQ_ERR_RANGE;
+  if (launch_ms) *launch_ms = 0.0;
+  if (count == 0) return DFQ_OK;
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(reserve_results(g, count));
+  HIP_TRY(hipMemcpyAsync(g->d_src, sources, sizeof(uint32_t) * count, hipMemcpyHostToDevice, g->stream));
+  BfsArgs a = make_args(g, sources[0], d_level, d_parent, false);
+  a.result = g->d_res;
+  const uint32_t *d_src = g->d_src;
+  void *args[] = {&a, &d_src, &count, &level_stride, &parent_stride};
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  HIP_TRY(hipLaunchCooperativeKernel(seq_kernel_for(g->cfg.block_size), dim3(g->grid_seq), dim3(g->block), args,
+                                     0, g->stream));
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpy(&err, &g->d_ctl->err, sizeof(err), hipMemcpyDeviceToHost));
+  if (err) return DFQ_ERR_CAPACITY;
+  std::vector<uint64_t> res((size_t)kResultWords * count);
+  HIP_TRY(hipMemcpy(res.data(), g->d_res, sizeof(uint64_t) * res.size(), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint64_t *r = &res[(size_t)kResultWords * i];
+    fill_stats(g, r, (double)(r[5] - r[4]) * 1e-6, g->grid_seq, stats[i]);
+  }
+  if (launch_ms) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    *launch_ms = ms;
   }
   return DFQ_OK;
 }

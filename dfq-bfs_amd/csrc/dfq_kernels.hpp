@@ -98,7 +98,8 @@ struct BfsArgs {
   uint32_t item_cap;     // items per segment (m / kItem + n bounds one level)
   Control *ctl;
   uint64_t *prof;        // [prof_levels][kProfFields] or null
-  uint64_t *result;      // [4]: depth, reached, bottom-up levels, edges; or null
+  uint64_t *result;      // [kResultWords] or null: depth, reached, bottom-up levels, edges,
+                         // device-clock ns at the search's start and end
   uint32_t prof_levels;
   uint32_t n;
   uint32_t n_words;
@@ -700,19 +701,28 @@ __device__ __forceinline__ uint64_t solo_sync(SharedAll<BLOCK> &sh, uint32_t cla
   return n;
 }
 
+// Set in the hand-back tag when the search ended during the solo levels.
+constexpr uint64_t kSoloDone = 1ull << 63;
+
 // The solo workgroup hands the search back (or reports its end): its XCD's
 // L2 is written back, the state goes to the control block, then the tag.
+// quick_end: the search has ended and the caller ends the kernel or passes
+// a grid barrier next, either of which writes the L2 back, so the tag goes
+// out at once (the others only leave).
 __device__ __forceinline__ void solo_publish(Control *ctl, uint64_t tag, uint32_t L, uint64_t n_cur,
                                              uint64_t n_prev, uint64_t m_f, uint64_t m_seen, uint64_t reached,
-                                             uint32_t bu_levels, bool done) {
+                                             uint32_t bu_levels, bool done, bool quick_end) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    l2_writeback();  // waits for this thread's stores; the others waited before the barrier
-    const uint64_t st[8] = {L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, done ? 1ull : 0ull};
-    for (int i = 0; i < 8; ++i)
-      __hip_atomic_store(&ctl->solo_state[i], st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    wait_own_stores();
-    __hip_atomic_store(&ctl->solo_rel.word, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(done && quick_end)) {
+      l2_writeback();  // waits for this thread's stores; the others waited before the barrier
+      const uint64_t st[7] = {L, n_cur, n_prev, m_f, m_seen, reached, bu_levels};
+      for (int i = 0; i < 7; ++i)
+        __hip_atomic_store(&ctl->solo_state[i], st[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      wait_own_stores();
+    }
+    __hip_atomic_store(&ctl->solo_rel.word, tag | (done ? kSoloDone : 0ull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
 }
@@ -725,9 +735,13 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
                                           uint64_t &reached, uint32_t &bu_levels) {
   __shared__ uint64_t st[8];
   if (threadIdx.x == 0) {
-    while (ld_relaxed(&ctl->solo_rel.word) != tag) __builtin_amdgcn_s_sleep(2);
-    l2_invalidate();
-    for (int i = 0; i < 8; ++i) st[i] = ld_relaxed(&ctl->solo_state[i]);
+    uint64_t w;
+    while (((w = ld_relaxed(&ctl->solo_rel.word)) & ~kSoloDone) != tag) __builtin_amdgcn_s_sleep(2);
+    st[7] = w & kSoloDone;
+    if (!st[7]) {
+      l2_invalidate();
+      for (int i = 0; i < 7; ++i) st[i] = ld_relaxed(&ctl->solo_state[i]);
+    }
   }
   __syncthreads();
   L = (uint32_t)st[0];
@@ -742,27 +756,12 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
   return done;
 }
 
-// One BFS over the whole grid (every workgroup must call it with the same
-// arguments): initialisation, then the level loop until a level claims no
-// vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
-template <int BLOCK>
-__device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team) {
-  using S = KernelShape<BLOCK>;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / kWave;
-  const uint32_t trank = sh.team_rank, tsize = sh.team_size;
-  const uint32_t gtid = trank * BLOCK + tid;
-  const uint32_t gstride = tsize * BLOCK;
-  const uint32_t gwave = trank * S::kWaves + wave;
-  const uint32_t nwaves = tsize * S::kWaves;
-  BarrierState &bst = team.bst;
-  Control *ctl = a.ctl;
-  if (tid < kProfFields) sh.prof[tid] = 0;
-
-  uint64_t solo_next = 0;  // tag of the next hand-back of solo levels (thread 0)
-  if (tid == 0) solo_next = ld_relaxed(&ctl->solo_rel.word) + 1;
-
-  // ---- initialisation (reference :216 cost init, done on the device) ----
+// Initial state of a search (reference :216 cost init, done on the
+// device), `gtid` of `gstride` threads: levels, visited bitmap and its
+// snapshot (isolated vertices premarked), frontier bitmaps, and the
+// level-0 frontier (the source alone, in segment 0 of buffer 0, whose
+// tails live in the slot of "level -1", slot 3).
+__device__ __forceinline__ void bfs_init(const BfsArgs &a, uint32_t gtid, uint32_t gstride) {
   for (uint32_t i = gtid; i < a.n; i += gstride) {
     a.level[i] = (i == a.source) ? 0u : kInf;
     if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
@@ -775,9 +774,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
   if (a.bottom_up)
     for (size_t w = gtid; w < 2 * fbs; w += gstride) a.fb[w] = 0;
-  // level-0 frontier: the source alone, in segment 0 of buffer 0, whose
-  // tails live in the slot of "level -1" (slot 3)
   if (gtid == 0) {
+    Control *ctl = a.ctl;
     a.qseg[0][0] = a.source;
     for (int s = 0; s < 4; ++s)
       for (int k = 0; k < kSegments; ++k) {
@@ -787,7 +785,49 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     ctl->slot[3].tail[0] = 1;
     ctl->err = 0;
   }
-  grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+}
+
+// Words of BfsArgs::result per search.
+constexpr uint32_t kResultWords = 8;
+
+// How a caller runs grid_bfs.  pre_init: bfs_init has already run over the
+// grid and been released (the first grid barrier is skipped), and
+// solo_seen is ctl->solo_rel as read before any workgroup could pass a
+// barrier of this search.  quick_end: see solo_publish.
+struct GridBfsMode {
+  bool pre_init;
+  bool quick_end;
+  uint64_t solo_seen;
+};
+
+// One BFS over the whole grid (every workgroup must call it with the same
+// arguments): initialisation, then the level loop until a level claims no
+// vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
+template <int BLOCK>
+__device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team,
+                                         GridBfsMode mode = GridBfsMode{false, false, 0}) {
+  using S = KernelShape<BLOCK>;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / kWave;
+  const uint32_t trank = sh.team_rank, tsize = sh.team_size;
+  const uint32_t gtid = trank * BLOCK + tid;
+  const uint32_t gstride = tsize * BLOCK;
+  const uint32_t gwave = trank * S::kWaves + wave;
+  const uint32_t nwaves = tsize * S::kWaves;
+  BarrierState &bst = team.bst;
+  Control *ctl = a.ctl;
+  if (tid < kProfFields) sh.prof[tid] = 0;
+
+  if (a.result && gtid == 0) a.result[4] = now_ns();
+  uint64_t solo_next = 0;  // tag of the next hand-back of solo levels (thread 0)
+  if (mode.pre_init) {
+    if (tid == 0) solo_next = (mode.solo_seen & ~kSoloDone) + 1;
+  } else {
+    if (tid == 0) solo_next = (ld_relaxed(&ctl->solo_rel.word) & ~kSoloDone) + 1;
+    bfs_init(a, gtid, gstride);
+    grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+  }
+  const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
 
   Visitor vis;
   vis.col = a.col;
@@ -847,7 +887,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     }
     const bool small = !bottom_up && tsize > 1 && n_cur <= kSoloMaxFrontier && m_f <= kSoloMaxEdges;
     if (solo && !small) {
-      solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, false);
+      solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, false, false);
       solo = false;
     } else if (!solo && small) {
       // tags count hand-backs: every workgroup read the last one before
@@ -990,9 +1030,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
           a.result[1] = reached;
           a.result[2] = bu_levels;
           a.result[3] = m_seen;
+          a.result[5] = now_ns();
         }
       }
-      if (solo) solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, true);
+      if (solo) solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, true, mode.quick_end);
       break;
     }
     n_prev = n_cur;
@@ -1027,11 +1068,48 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
   GridBarrier *bar = &a.ctl->bar;
+  // One search per launch: its initial state is written while the census
+  // gathers the grid, each workgroup its blockIdx share, and released by
+  // the search's first grid barrier.  solo_rel is read before this
+  // workgroup is counted, so before any hand-back of this search.
+  uint64_t solo_seen = 0;
+  if (threadIdx.x == 0) solo_seen = ld_relaxed(&a.ctl->solo_rel.word);
+  bfs_init(a, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
   if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->err = 2;
     return;
   }
-  grid_bfs<BLOCK>(a, sh, team);
+  grid_barrier(bar, team.bst, 0, sh.bcast);
+  // the kernel ends after the search, which writes every L2 back
+  grid_bfs<BLOCK>(a, sh, team, GridBfsMode{true, true, solo_seen});
+}
+
+// Many searches of one graph in one launch, in turn: search i starts from
+// sources[i] and writes level + i * level_stride (and parent + i *
+// parent_stride when parents are asked for) and result + i * kResultWords.
+// Each search's first grid barrier also ends the previous one.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(BfsArgs a, const uint32_t *sources, uint32_t count,
+                                                            uint64_t level_stride, uint64_t parent_stride) {
+  __shared__ SharedAll<BLOCK> sh;
+  Team team{};
+  GridBarrier *bar = &a.ctl->bar;
+  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->err = 2;
+    return;
+  }
+  uint32_t *const level0 = a.level;
+  uint32_t *const parent0 = a.parent;
+  uint64_t *const result0 = a.result;
+  for (uint32_t i = 0; i < count; ++i) {
+    a.source = sources[i];
+    a.level = level0 + i * level_stride;
+    a.parent = parent0 ? parent0 + i * parent_stride : nullptr;
+    a.result = result0 + (size_t)i * kResultWords;
+    // the last search ends with the kernel; the others are followed by the
+    // next one's initialisation, which needs their stores written back
+    grid_bfs<BLOCK>(a, sh, team, GridBfsMode{false, i + 1 == count, 0});
+  }
 }
 
 }  // namespace dfq

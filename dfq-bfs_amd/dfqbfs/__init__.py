@@ -142,6 +142,8 @@ _sig("dfq_graph_device_arrays", C.c_int, _P, C.POINTER(_P), C.POINTER(_P))
 _sig("dfq_bfs", C.c_int, _P, C.c_uint32, _u32p, _u32p, C.POINTER(Stats))
 _sig("dfq_bfs_device", C.c_int, _P, C.c_uint32, _P, _P, C.POINTER(Stats))
 _sig("dfq_bfs_device_many", C.c_int, _P, C.c_uint32, _u32p, _P, _P, C.POINTER(Stats))
+_sig("dfq_bfs_device_seq", C.c_int, _P, C.c_uint32, _u32p, _P, C.c_uint64, _P, C.c_uint64,
+     C.POINTER(Stats), C.POINTER(C.c_double))
 _sig("dfq_bfs_profile", C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32,
      C.POINTER(Stats))
 _sig("dfq_graph_is_symmetric", C.c_int, _P)
@@ -389,6 +391,17 @@ class DeviceGraph:
         check(lib.dfq_bfs_device_many(self._h, so.size, _u32(so), d_level, d_parent, st),
               "dfq_bfs_device_many")
         return list(st[:so.size])
+
+    def bfs_device_seq(self, sources, d_level: int, level_stride: int = 0,
+                       d_parent: int | None = None, parent_stride: int = 0):
+        """Searches from every source in one launch; returns (per-search
+        Stats with device-clock times, event time of the launch in ms)."""
+        so = np.ascontiguousarray(sources, np.uint32)
+        st = (Stats * max(so.size, 1))()
+        ms = C.c_double()
+        check(lib.dfq_bfs_device_seq(self._h, so.size, _u32(so), d_level, level_stride, d_parent,
+                                     parent_stride, st, C.byref(ms)), "dfq_bfs_device_seq")
+        return list(st[:so.size]), ms.value
 
     def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
         """BFS into caller-owned device buffers (raw device addresses)."""

@@ -134,6 +134,21 @@ int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
 int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources,
                         uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats);
 
+/* count searches in ONE cooperative launch: the grid runs them in turn,
+ * each one's initialisation barrier also closing the one before, so the
+ * launch cost is paid once (the reference's interactive loop,
+ * bfs.cu:202-214, relaunched the kernel per source).  Search i writes
+ * d_level + i * level_stride and, when d_parent is given, d_parent +
+ * i * parent_stride; a stride of 0 makes every search overwrite the same
+ * array (else it must be >= n).  stats[i].kernel_ms is search i's
+ * device-clock time inside the launch (its initialisation to its last
+ * level); *launch_ms (optional) is the hipEvent time of the whole launch.
+ * Same results as count dfq_bfs_device calls. */
+int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources,
+                       uint32_t *d_level, uint64_t level_stride,
+                       uint32_t *d_parent, uint64_t parent_stride,
+                       dfq_stats *stats, double *launch_ms);
+
 /* BFS with per-level counters (the reference's frontier profiling build,
  * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,140-144,
  * 200-209).  rows[max_levels][DFQ_PROFILE_FIELDS], one row per level:

@@ -240,3 +240,64 @@ def test_device_many_back_to_back(dfq, oracle, direction_opt):
         np.testing.assert_array_equal(lv.to_numpy(), oracle.bfs(csr.row_offsets, csr.col_idx, srcs[-1]))
         with pytest.raises(dfq.DfqError):
             g.bfs_device_many([csr.n], lv.ptr)
+
+
+SEQ_GRAPHS = [
+    ("rmat14", lambda dfq: dfq.Csr.rmat(14, 16, seed=78)),
+    ("grid256", lambda dfq: dfq.Csr.grid2d(256, 256, 0.05, seed=11)),   # long solo-level runs
+    ("amazon", lambda dfq: dfq.Csr.amazon_like(65536, seed=5)),
+]
+
+
+@pytest.mark.parametrize("direction_opt", [0, 1])
+@pytest.mark.parametrize("gname", [g[0] for g in SEQ_GRAPHS])
+def test_device_seq_one_launch(dfq, oracle, gname, direction_opt):
+    """dfq_bfs_device_seq: every search of one launch equals the oracle
+    (levels kept apart with stride n, parents valid), stats match, and a
+    stride of 0 leaves the last search's levels."""
+    csr = dict(SEQ_GRAPHS)[gname](dfq)
+    n = csr.n
+    deg = np.diff(csr.row_offsets)
+    srcs = [int(s) for s in csr.pick_sources(9, 17)]
+    iso = np.flatnonzero(deg == 0)
+    if iso.size:
+        srcs.insert(3, int(iso[0]))         # isolated source mid-sequence
+    srcs.append(srcs[0])                     # a repeated source
+    cnt = len(srcs)
+    lv = dfq.DeviceBuffer(4 * n * cnt)
+    pa = dfq.DeviceBuffer(4 * n * cnt)
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(direction_opt=direction_opt)
+        say("seq", gname, direction_opt, cnt)
+        stats, launch_ms = g.bfs_device_seq(srcs, lv.ptr, n, pa.ptr, n)
+        assert launch_ms > 0 and len(stats) == cnt
+        levels = lv.to_numpy().reshape(cnt, n)
+        parents = pa.to_numpy().reshape(cnt, n)
+        for i, (s, st) in enumerate(zip(srcs, stats)):
+            e = oracle.bfs(csr.row_offsets, csr.col_idx, s)
+            np.testing.assert_array_equal(levels[i], e, err_msg=f"search {i} source {s}")
+            rep = csr.validate(s, levels[i], parents[i])
+            assert rep["errors"] == 0, (i, s, rep)
+            assert st.reached == int((e != UNREACHED).sum())
+            assert st.depth == int(e[e != UNREACHED].max()) + 1
+            assert st.kernel_ms > 0
+        assert sum(st.kernel_ms for st in stats) <= launch_ms * 1.05 + 0.05
+        # stride 0: the searches overwrite one array; the last one stays
+        one = dfq.DeviceBuffer(4 * n)
+        g.bfs_device_seq(srcs[::-1], one.ptr)
+        np.testing.assert_array_equal(one.to_numpy(), levels[0])
+        # a plain launch afterwards shares no state with the sequence
+        r = g.bfs(srcs[1])
+        np.testing.assert_array_equal(r.level, levels[1])
+
+
+def test_device_seq_bad_input(dfq):
+    csr = dfq.Csr.rmat(10, 8, seed=2)
+    lv = dfq.DeviceBuffer(4 * csr.n * 2)
+    with dfq.DeviceGraph(csr) as g:
+        with pytest.raises(dfq.DfqError):
+            g.bfs_device_seq([0, 1], lv.ptr, csr.n - 1)      # stride below n
+        with pytest.raises(dfq.DfqError):
+            g.bfs_device_seq([0, csr.n], lv.ptr, csr.n)      # source out of range
+        st, ms = g.bfs_device_seq([], lv.ptr)
+        assert st == [] and ms == 0.0

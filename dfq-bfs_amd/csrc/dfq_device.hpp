@@ -59,6 +59,24 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Exclusive prefix of the kSegments tails into pre[0..kSegments], the total
+// in pre[kSegments].  Wave 0 only: lanes 0..kSegments-1 load one tail each,
+// so the tails cost one memory round trip instead of a chain of them.
+template <int kN>
+__device__ __forceinline__ void tails_prefix(const uint32_t *tails, uint32_t *pre) {
+  static_assert(kN <= kWave && (kN & (kN - 1)) == 0, "tail count");
+  const uint32_t lane = lane_id();
+  const uint32_t t = lane < (uint32_t)kN ? ld_agent(&tails[lane]) : 0u;
+  uint32_t x = t;
+#pragma unroll
+  for (int d = 1; d < kN; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane < (uint32_t)kN) pre[lane] = x - t;
+  if (lane == (uint32_t)kN - 1) pre[kN] = x;
+}
+
 // ---------------------------------------------------------------------------
 // Grid barrier, XCD-aware.
 //
@@ -93,10 +111,10 @@ struct alignas(256) BarrierWord {
 
 struct GridBarrier {
   BarrierWord xleaf[kMaxXcd];   // per-XCD arrivals
-  BarrierWord xside[kMaxXcd];   // per-XCD sum of the 64-bit side payload
+  BarrierWord xside[kMaxXcd];   // per-XCD side arrivals: count (bits 40..63) | side sum
   BarrierWord root;             // XCD-leader arrivals
-  BarrierWord root_side;        // side payload sum over XCDs
-  BarrierWord side_out;         // side payload total of the last round
+  BarrierWord root_side;        // XCD side arrivals: count | side sum
+  BarrierWord side_out;         // generation (bits 40..63) | side total of that round
   BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
   BarrierWord census_epoch;     // launches counted so far (low bit picks the census set)
   BarrierWord ticket;           // workgroup ranks handed out within a team
@@ -223,69 +241,89 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
   __hip_atomic_store(&tb->ticket.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// All threads of every workgroup must call it.  `payload` (< 2^40) and
-// `side` are read from thread 0; returns the grid-wide payload sum in every
-// thread and, when side_total is given, the side sum there.  `st` is only
-// meaningful in thread 0.  `s_word` points at two 64-bit LDS slots.
+// All threads of every workgroup must call it.  `payload` and `side` (each
+// summing to < 2^40 over the grid) are read from thread 0; returns the
+// grid-wide payload sum in every thread and, when side_total is given (by
+// every workgroup alike), the side sum there.  `st` is only meaningful in
+// thread 0.  `s_word` points at two 64-bit LDS slots.
 // kSystem: the XCD leaders write their L2 back at system scope, so every
 // store made before the barrier can be read by the host or a DMA copy.
+//
+// The payload and the side sum travel in two arrival words each carrying
+// its own count, added together by every workgroup (one round trip).  Per
+// XCD the last arrival on the payload word writes the L2 back and arrives
+// at the root's payload word; the last arrival on the side word forwards
+// its sum to the root's side word (the two may be different workgroups).
+// The last root arrivals publish generation | total in gensum and
+// side_out; every workgroup polls both words in one round trip per poll.
 template <bool kSystem = false>
 __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &st, uint64_t payload,
                                                  uint64_t *s_word, uint64_t side = 0,
                                                  uint64_t *side_total = nullptr) {
+  const bool with_side = side_total != nullptr;
   wait_own_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t g = st.gen, next = (g + 1) & 0xffffffu;
-    if (side) {
-      // lands before this workgroup's arrival is counted
-      __hip_atomic_fetch_add(&b->xside[st.xcc].word, side, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      wait_own_stores();
-    }
+    const uint32_t next = (st.gen + 1) & 0xffffffu;
+    const uint64_t tag = (uint64_t)next << 40;
+    // both arrivals in flight together
+    uint64_t os = 0;
+    if (with_side)
+      os = __hip_atomic_fetch_add(&b->xside[st.xcc].word, kArrive | side, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t old = __hip_atomic_fetch_add(&b->xleaf[st.xcc].word, kArrive | payload,
                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint64_t total = 0;
-    bool released = false;
-    if ((old >> 40) + 1 == st.xcc_n) {
-      // XCD leader: every workgroup of this XCD has its stores in this L2
-      const uint64_t xsum = (old & kPayloadMask) + payload;
-      const uint64_t xs = ld_relaxed(&b->xside[st.xcc].word);
-      __hip_atomic_store(&b->xleaf[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&b->xside[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (xs) __hip_atomic_fetch_add(&b->root_side.word, xs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // also waits for the stores and atomics above
+    // last arrivals of this XCD: on the side word it forwards the XCD's side
+    // sum; on the payload word it writes the L2 back (every workgroup of the
+    // XCD has its stores there) and forwards the payload sum.  One
+    // workgroup may be both; its two root arrivals then go out together.
+    const bool lead_s = with_side && (os >> 40) + 1 == st.xcc_n;
+    const bool lead_p = (old >> 40) + 1 == st.xcc_n;
+    const uint64_t xs = (os & kPayloadMask) + side;
+    const uint64_t xsum = (old & kPayloadMask) + payload;
+    if (lead_s) __hip_atomic_store(&b->xside[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lead_p) __hip_atomic_store(&b->xleaf[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the resets land before this XCD can count as arrived at the root
+    if (lead_p) {
       if (kSystem) l2_writeback_system();
       else l2_writeback();
-      const uint64_t o2 = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-      if ((o2 >> 40) + 1 == st.nx) {
-        total = (o2 & kPayloadMask) + xsum;
-        const uint64_t rs = ld_relaxed(&b->root_side.word);
-        __hip_atomic_store(&b->root.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&b->root_side.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&b->side_out.word, rs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        wait_own_stores();
-        __hip_atomic_store(&b->gensum.word, ((uint64_t)next << 40) | total, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        released = true;
-      }
+    } else if (lead_s) {
+      wait_own_stores();
     }
-    if (!released) {
-      // every workgroup waits on the root's word itself (no per-XCD relay)
-      uint64_t w;
-      while (((w = ld_relaxed(&b->gensum.word)) >> 40) != next) __builtin_amdgcn_s_sleep(1);
-      total = w & kPayloadMask;
+    uint64_t rs = 0, rp = 0;
+    if (lead_s)
+      rs = __hip_atomic_fetch_add(&b->root_side.word, kArrive | xs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lead_p)
+      rp = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool fin_s = lead_s && (rs >> 40) + 1 == st.nx;
+    const bool fin_p = lead_p && (rp >> 40) + 1 == st.nx;
+    if (fin_s) __hip_atomic_store(&b->root_side.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (fin_p) __hip_atomic_store(&b->root.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (fin_s || fin_p) wait_own_stores();
+    if (fin_s)
+      __hip_atomic_store(&b->side_out.word, tag | ((rs & kPayloadMask) + xs), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (fin_p)
+      __hip_atomic_store(&b->gensum.word, tag | ((rp & kPayloadMask) + xsum), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    // every workgroup waits on the root's words themselves (no relay)
+    uint64_t w, ws = tag;
+    for (;;) {
+      w = ld_relaxed(&b->gensum.word);
+      if (with_side) ws = ld_relaxed(&b->side_out.word);
+      if ((w >> 40) == next && (ws >> 40) == next) break;
+      __builtin_amdgcn_s_sleep(1);
     }
     // this CU's L1 may hold stale lines; on gfx950 only the device-scope
     // invalidate reaches it (workgroup scope leaves L1 alone outside
     // thread-group-split mode), and it also drops this XCD's L2 lines
     l2_invalidate();
     st.gen = next;
-    s_word[0] = total;
-    if (side_total) s_word[1] = ld_relaxed(&b->side_out.word);
+    s_word[0] = w & kPayloadMask;
+    s_word[1] = ws & kPayloadMask;
   }
   __syncthreads();
-  if (side_total) *side_total = s_word[1];
+  if (with_side) *side_total = s_word[1];
   return s_word[0];
 }
 

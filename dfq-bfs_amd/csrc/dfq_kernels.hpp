@@ -336,13 +336,8 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
                                 uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
+  if (tid < kWave) tails_prefix<kSegments>(in_slot.tail, sh.seg_pre);
   if (tid == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kSegments; ++i) {
-      sh.seg_pre[i] = t;
-      t += ld_agent(&in_slot.tail[i]);
-    }
-    sh.seg_pre[kSegments] = t;
     sh.med_n = 0;
     sh.big_n = 0;
   }
@@ -452,14 +447,7 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
                               uint64_t &edges) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
   const uint32_t tid = threadIdx.x, lane = lane_id();
-  if (tid == 0) {
-    uint32_t t = 0;
-    for (int i = 0; i < kSegments; ++i) {
-      sh.seg_pre[i] = t;
-      t += ld_agent(&in_slot.tail[i]);
-    }
-    sh.seg_pre[kSegments] = t;
-  }
+  if (tid < kWave) tails_prefix<kSegments>(in_slot.tail, sh.seg_pre);
   __syncthreads();
   const uint32_t total = sh.seg_pre[kSegments];
   uint32_t v = 0;
@@ -734,13 +722,18 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
                                           uint64_t &n_cur, uint64_t &n_prev, uint64_t &m_f, uint64_t &m_seen,
                                           uint64_t &reached, uint32_t &bu_levels) {
   __shared__ uint64_t st[8];
-  if (threadIdx.x == 0) {
-    uint64_t w;
-    while (((w = ld_relaxed(&ctl->solo_rel.word)) & ~kSoloDone) != tag) __builtin_amdgcn_s_sleep(2);
-    st[7] = w & kSoloDone;
+  if (threadIdx.x < kWave) {
+    const uint32_t lane = lane_id();
+    if (lane == 0) {
+      uint64_t w;
+      while (((w = ld_relaxed(&ctl->solo_rel.word)) & ~kSoloDone) != tag) __builtin_amdgcn_s_sleep(2);
+      st[7] = w & kSoloDone;
+    }
+    wave_sync();
     if (!st[7]) {
-      l2_invalidate();
-      for (int i = 0; i < 7; ++i) st[i] = ld_relaxed(&ctl->solo_state[i]);
+      if (lane == 0) l2_invalidate();
+      // the seven state words in one request
+      if (lane < 7) st[lane] = ld_relaxed(&ctl->solo_state[lane]);
     }
   }
   __syncthreads();
@@ -952,14 +945,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
       }
       m_f = mf1;
-      if (tid == 0) {
-        uint32_t t = 0;
-        for (int k = 0; k < kSegments; ++k) {
-          sh.item_pre[k] = t;
-          t += ld_agent(&cur.item_tail[k]);
-        }
-        sh.item_pre[kSegments] = t;
-      }
+      if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
       __syncthreads();
       hc = sh.item_pre[kSegments];
       if (hc) {

@@ -339,25 +339,14 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
 constexpr int kSegments = 8;
 constexpr uint32_t kStage = 1024;  // per-wave LDS staging entries
 
-// Visited-bitmap claim: a plain load filters vertices already seen (a stale
-// copy can only say "unvisited", never the reverse, because bits are only
-// set during a search); the atomic OR decides ownership.
-__device__ __forceinline__ bool claim_vertex(uint32_t *vis, uint32_t w, uint32_t &atomics) {
-  const uint32_t word = w >> 5, bit = 1u << (w & 31u);
-  if (vis[word] & bit) return false;
-  ++atomics;
-  const uint32_t old = __hip_atomic_fetch_or(&vis[word], bit, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-  return (old & bit) == 0;
-}
-
 // Per-thread state of the edge visitor; out_n / claims are wave-uniform.
 struct Visitor {
   const uint32_t *__restrict__ col;
   uint32_t *level;
   uint32_t *parent;     // may be null
   uint32_t *vis;
-  const uint32_t *snap; // visited bitmap as of the level's start (subset of vis)
+  const uint32_t *snap; // visited bitmap as of the level's start (subset of vis), or vis itself
+  bool probe_live;      // snap is a snapshot: probe the live bitmap after it
   uint32_t *seg;        // output segment of this workgroup's XCD
   uint32_t *seg_tail;   // its tail
   uint32_t *stage;      // this wave's LDS staging buffer [kStage]
@@ -399,17 +388,21 @@ struct Visitor {
   //
   // Filter order: the level-start snapshot of the visited bitmap (never
   // written during the level, so its lines stay in L1/L2), then the live
-  // bitmap for the few neighbours unvisited at level start, then the atomic.
-  // Both loads can only under-report visited vertices, never over-report.
+  // bitmap for the few neighbours unvisited at level start (probe_live),
+  // then the atomic.  On levels without a snapshot `snap` is the live
+  // bitmap and is probed once.  The loads can only under-report visited
+  // vertices, never over-report.
   template <int U>
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
                                               const uint32_t (&w)[U]) {
     uint32_t word[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) word[i] = act[i] ? snap[w[i] >> 5] : 0xffffffffu;
+    if (probe_live) {
 #pragma unroll
-    for (int i = 0; i < U; ++i)
-      if (!((word[i] >> (w[i] & 31u)) & 1u)) word[i] = vis[w[i] >> 5];
+      for (int i = 0; i < U; ++i)
+        if (!((word[i] >> (w[i] & 31u)) & 1u)) word[i] = vis[w[i] >> 5];
+    }
     uint32_t old[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) {
@@ -438,25 +431,6 @@ struct Visitor {
       const uint64_t mask = __ballot(claimed);
       if (mask) append(claimed, mask, w[i]);
     }
-  }
-
-  // Called by every lane of the wave (wave-uniform control flow); `act`
-  // marks lanes that carry an edge u -> w.
-  __device__ __forceinline__ void visit(bool act, uint32_t u, uint32_t w) {
-    bool claimed = false;
-    if (act) {
-      ++probes;
-      claimed = claim_vertex(vis, w, atomics);
-      if (claimed) {
-        level[w] = next_level;
-        if (parent) parent[w] = u;
-        if (track_mf) mf += (uint64_t)(off[w + 1] - off[w]);
-        if (fb_mark)
-          __hip_atomic_fetch_or(&fb_mark[w >> 5], 1u << (w & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    const uint64_t mask = __ballot(claimed);
-    if (mask) append(claimed, mask, w);
   }
 };
 

@@ -671,6 +671,10 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
   }
 }
 
+// A top-down level takes a visited-bitmap snapshot when its frontier has at
+// least N / kSnapshotDiv out-edges.
+constexpr uint64_t kSnapshotDiv = 16;
+
 // Solo levels: at most this many frontier vertices and out-edges.
 constexpr uint64_t kSoloMaxFrontier = 1024;
 constexpr uint64_t kSoloMaxEdges = 8192;
@@ -827,7 +831,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   vis.level = a.level;
   vis.parent = a.parent;
   vis.vis = a.vis;
-  vis.snap = a.vis_snap;
+  vis.snap = a.vis;
+  vis.probe_live = false;
   vis.stage = sh.stage[wave];
   vis.out_n = 0;
   vis.probes = 0;
@@ -911,21 +916,28 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     if (a.prof && tid == 0) t_start = now_ns();
     if (!bottom_up) {
       bool fb_track = false;
-      if (solo) {
-        // the snapshot of the last team level stays a valid (older) filter;
-        // the frontier bitmaps are not touched
-        vis.fb_mark = nullptr;
-      } else {
-        // refresh the snapshot: words copied while claims go on are still
-        // subsets of the visited set, which is all the filter needs
+      vis.fb_mark = nullptr;
+      // A level with many out-edges filters through a snapshot of the
+      // visited bitmap taken at its start (read-only while the atomics hit
+      // the live words); words copied while claims go on are still subsets
+      // of the visited set, which is all the filter needs.  Smaller levels
+      // (and solo ones) probe the live bitmap alone.
+      const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n;
+      vis.snap = snapshot ? a.vis_snap : a.vis;
+      vis.probe_live = snapshot;
+      if (snapshot)
         for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
-        if (a.bottom_up) {
-          uint32_t *fb_here = a.fb + (L & 1) * fbs;
-          for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
-          fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
-          vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
-          fb_zero = 1u << (L & 1);
-        }
+      // A growing frontier may turn the next level bottom-up: zero this
+      // level's bitmap parity for later and OR the claims into the next
+      // one when it is known to be zero.  Solo levels and shrinking
+      // frontiers leave the bitmaps alone (a bottom-up level after them
+      // builds its bitmap from the level array).
+      if (a.bottom_up && !solo && n_cur > n_prev) {
+        uint32_t *fb_here = a.fb + (L & 1) * fbs;
+        for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
+        fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
+        vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
+        fb_zero = 1u << (L & 1);
       }
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the

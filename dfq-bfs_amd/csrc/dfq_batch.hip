@@ -58,6 +58,7 @@ struct dfq_batch {
   int wg_queue_grid = 0;
   uint32_t *d_iso = nullptr;  // isolated-vertex bitmaps of the symmetric graphs
   uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs' ranges)
+  uint4 *d_head = nullptr;       // first four col_bu entries per vertex, indexed like d_off
   // launch state of the current tables
   BatchArgs args{};
   bool tables_ok = false;
@@ -91,7 +92,7 @@ void free_all(dfq_batch *b) {
   drop_graph_exec(b);
   void *ptrs[] = {b->d_off, b->d_col, b->d_graphs, b->d_level, b->d_queries, b->d_results,
                   b->d_vis, b->d_vis_snap, b->d_fb, b->d_qseg[0], b->d_qseg[1], b->d_items,
-                  b->d_ctl, b->d_work, b->d_wg_queue, b->d_iso, b->d_col_bu};
+                  b->d_ctl, b->d_work, b->d_wg_queue, b->d_iso, b->d_col_bu, b->d_head};
   for (void *p : ptrs) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -196,6 +197,7 @@ int build_tables(dfq_batch *b, const std::vector<uint32_t> &subset) {
   a.wg_queue = b->d_wg_queue;
   a.iso = b->d_iso;
   a.col_bu = b->d_col_bu;
+  a.head = b->d_head;
   BfsArgs &s = a.grid;
   s.vis = b->d_vis;
   s.vis_snap = b->d_vis_snap;
@@ -356,7 +358,8 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
   {
     // symmetric graphs: a second copy of the rows by neighbour degree for
     // bottom-up levels, and isolated vertices marked visited from the start
-    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered;
+    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered, heads;
+    BFAIL(hipMalloc(&b->d_head, sizeof(uint4) * vtot));
     for (uint32_t g = 0; g < num_graphs; ++g) {
       const BatchGraph &gr = graphs[g];
       const int64_t *off = row_offsets + gr.vbase;
@@ -368,6 +371,9 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
       isolated_bitmap(off, gr.n, iso.data() + gr.iso_base);
       if (gr.m)
         BFAIL(hipMemcpy(b->d_col_bu + gr.ebase, ordered.data(), sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+      heads.resize(4 * (size_t)gr.n);
+      row_heads(off, ordered.data(), gr.n, heads.data());
+      if (gr.n) BFAIL(hipMemcpy(b->d_head + gr.vbase, heads.data(), sizeof(uint4) * gr.n, hipMemcpyHostToDevice));
     }
     BFAIL(hipMalloc(&b->d_iso, sizeof(uint32_t) * iso.size()));
     BFAIL(hipMemcpy(b->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));

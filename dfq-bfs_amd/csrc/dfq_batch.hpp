@@ -55,6 +55,7 @@ struct BatchArgs {
   const int64_t *off;         // concatenated row offsets
   const uint32_t *col;        // concatenated columns
   const uint32_t *col_bu;     // the same, rows by neighbour degree (symmetric graphs)
+  const uint4 *head;          // first four col_bu entries per vertex, indexed like off
   const BatchGraph *graphs;
   const BatchQuery *queries;  // grid-tier queries first, then workgroup tier
   BatchResult *results;
@@ -380,6 +381,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     a.off = b.off + gr.vbase;
     a.col = b.col + gr.ebase;
     a.col_bu = gr.symmetric ? b.col_bu + gr.ebase : nullptr;
+    a.head = gr.symmetric ? b.head + gr.vbase : nullptr;
     a.level = b.level + q.lbase;
     a.parent = nullptr;
     a.n = gr.n;

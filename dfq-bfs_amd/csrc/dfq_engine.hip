@@ -64,6 +64,7 @@ struct dfq_graph {
   uint32_t *d_fb = nullptr;
   uint32_t *d_iso = nullptr;     // isolated vertices (symmetric graphs only)
   uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs only)
+  uint4 *d_head = nullptr;       // first four entries of every col_bu row (symmetric graphs only)
   uint2 *d_items = nullptr;
   uint32_t item_cap = 0;
   uint32_t *d_qseg[2] = {nullptr, nullptr};  // kSegments x n entries each
@@ -145,6 +146,7 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_fb);
   (void)hipFree(g->d_iso);
   (void)hipFree(g->d_col_bu);
+  (void)hipFree(g->d_head);
   (void)hipFree(g->d_items);
   (void)hipFree(g->d_prof);
   (void)hipFree(g->d_ctl);
@@ -167,6 +169,7 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.fb = g->d_fb;
   a.iso = g->d_iso;
   a.col_bu = g->d_col_bu;
+  a.head = g->d_head;
   a.result = nullptr;
   a.qseg[0] = g->d_qseg[0];
   a.qseg[1] = g->d_qseg[1];
@@ -362,6 +365,17 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
     isolated_bitmap(row_offsets, n, iso.data());
     HIP_OR_FAIL(hipMalloc(&g->d_col_bu, sizeof(uint32_t) * (m + 4)));
     if (m) HIP_OR_FAIL(hipMemcpy(g->d_col_bu, ordered.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    {
+      std::vector<uint32_t> heads;
+      try {
+        heads.resize(4 * (size_t)n);
+      } catch (...) {
+        return fail(DFQ_ERR_NOMEM);
+      }
+      row_heads(row_offsets, ordered.data(), n, heads.data());
+      HIP_OR_FAIL(hipMalloc(&g->d_head, sizeof(uint4) * (size_t)n));
+      HIP_OR_FAIL(hipMemcpy(g->d_head, heads.data(), sizeof(uint4) * (size_t)n, hipMemcpyHostToDevice));
+    }
     HIP_OR_FAIL(hipMalloc(&g->d_iso, sizeof(uint32_t) * g->n_words));
     HIP_OR_FAIL(hipMemcpy(g->d_iso, iso.data(), sizeof(uint32_t) * g->n_words, hipMemcpyHostToDevice));
   }

@@ -87,6 +87,7 @@ struct BfsArgs {
   const int64_t *off;    // row offsets [n+1]
   const uint32_t *col;   // column indices [m] (top-down levels read these)
   const uint32_t *col_bu; // the same rows ordered by neighbour degree (bottom-up levels), or null
+  const uint4 *head;     // first four entries of every col_bu row [n] (bottom-up levels), or null
   uint32_t *level;       // [n]
   uint32_t *parent;      // [n] or null
   const uint32_t *iso;   // isolated vertices [n_words] (symmetric graphs) or null
@@ -559,10 +560,11 @@ __device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t
 
 // One bottom-up level.  Each wave takes chunks of kBuWords visited words:
 // it lists the chunk's unvisited vertices in LDS, then every lane streams
-// through its share of that list as a three-stage pipeline — row offsets,
-// kUnroll column indices, their frontier-bitmap words — so that each loop
-// trip costs one memory round trip for the whole wave whatever stage each
-// lane is in, and a lane that finishes a vertex starts the next at once.
+// through its share of that list as a three-stage pipeline — row offsets
+// (with the row's first four entries from the head array when there is
+// one), four column indices, their frontier-bitmap words — so that each
+// loop trip costs one memory round trip for the whole wave whatever stage
+// each lane is in, and a lane that finishes a vertex starts the next at once.
 // The first frontier neighbour in row order becomes the parent.  Claims are
 // collected in LDS; the wave then writes the chunk's visited words and the
 // next level's frontier-bitmap words itself (it owns them: no atomics).
@@ -572,6 +574,8 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
   const uint32_t lane = lane_id();
   const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
+  const uint4 *head = a.head;
+  const bool hd = head != nullptr;
   uint32_t *cand = sh.u.cand[wave];
   uint32_t *cm = sh.bu_claim[wave];
   const uint32_t chunks = (a.n_words + kBuWords - 1) / kBuWords;
@@ -612,11 +616,15 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
       const int64_t *po = a.off + (st0 ? v : 0u);
       const uint32_t *pk = colb + (st1 ? k : 0);
       const uint32_t back = (uint32_t)(((uintptr_t)pk & 15) >> 2);
+      // with the head array, stage 0 fetches the row's first block with its
+      // offsets: neighbouring candidates' heads are neighbouring 16-byte
+      // words, so the trip's head loads are a few whole lines
+      const uint4 *pq = (st0 && hd) ? head + v : (const uint4 *)(pk - back);
       uint32_t fi[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) fi[i] = (st2 && kb + i >= k && kb + i < e) ? (cw[i] >> 5) : 0u;
       const int64_t o0 = po[0], o1 = po[1];
-      const uint4 q = *(const uint4 *)(pk - back);  // derived from col: a global load
+      const uint4 q = *pq;  // derived from col or head: a global load
       uint32_t fw[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) fw[i] = fb_cur[fi[i]];
@@ -636,13 +644,14 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
       vis.probes += scanned;
       edges += scanned;
       const uint32_t next2 = (!hit && kn < e) ? 1u : 0u;
-      const uint32_t next0 = o1 > o0 ? 1u : 0u;
+      const uint32_t next0 = o1 > o0 ? (hd ? 2u : 1u) : 0u;
       stage = st0 ? next0 : (st1 ? 2u : (st2 ? next2 : stage));
-      cw[0] = st1 ? q.x : cw[0];
-      cw[1] = st1 ? q.y : cw[1];
-      cw[2] = st1 ? q.z : cw[2];
-      cw[3] = st1 ? q.w : cw[3];
-      kb = st1 ? k - (int64_t)back : kb;  // CSR index of cw[0]
+      const bool take = st1 || (st0 && hd);
+      cw[0] = take ? q.x : cw[0];
+      cw[1] = take ? q.y : cw[1];
+      cw[2] = take ? q.z : cw[2];
+      cw[3] = take ? q.w : cw[3];
+      kb = st1 ? k - (int64_t)back : ((st0 && hd) ? o0 : kb);  // CSR index of cw[0]
       const int64_t deg = e - s0;
       k = st0 ? o0 : (st2 ? kn : k);
       s0 = st0 ? o0 : s0;

@@ -127,6 +127,7 @@ struct dfq_server {
   ServeGraph *d_graphs = nullptr;
   uint32_t *d_iso = nullptr;
   uint32_t *d_col_bu = nullptr;  // rows by neighbour degree (symmetric graphs' ranges)
+  uint4 *d_head = nullptr;       // first four col_bu entries per vertex, indexed like d_off
   ServeCtl *d_ctl = nullptr;
   uint8_t *d_arena = nullptr;
   // pinned, host-mapped
@@ -188,7 +189,7 @@ bool kernel_ended(dfq_server *s) { return hipStreamQuery(s->stream) != hipErrorN
 
 void free_all(dfq_server *s) {
   if (s->wedged) return;  // a launch may still read this memory
-  void *dev[] = {s->d_off, s->d_col, s->d_col_bu, s->d_graphs, s->d_iso, s->d_ctl, s->d_arena};
+  void *dev[] = {s->d_off, s->d_col, s->d_col_bu, s->d_head, s->d_graphs, s->d_iso, s->d_ctl, s->d_arena};
   for (void *p : dev) (void)hipFree(p);
   if (s->h_block) (void)hipHostFree(s->h_block);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -394,7 +395,8 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   {
     // symmetric graphs: a second copy of the rows by neighbour degree for
     // bottom-up levels, and isolated vertices marked visited from the start
-    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered;
+    std::vector<uint32_t> iso(iso_words ? iso_words : 1), ordered, heads;
+    SFAIL(hipMalloc(&s->d_head, sizeof(uint4) * vtot));
     for (uint32_t g = 0; g < num_graphs; ++g) {
       const ServeGraph &gr = graphs[g];
       const int64_t *off = row_offsets + gr.vbase;
@@ -406,6 +408,9 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
       isolated_bitmap(off, gr.n, iso.data() + gr.iso_base);
       if (gr.m)
         SFAIL(hipMemcpy(s->d_col_bu + gr.ebase, ordered.data(), sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+      heads.resize(4 * (size_t)gr.n);
+      row_heads(off, ordered.data(), gr.n, heads.data());
+      if (gr.n) SFAIL(hipMemcpy(s->d_head + gr.vbase, heads.data(), sizeof(uint4) * gr.n, hipMemcpyHostToDevice));
     }
     SFAIL(hipMalloc(&s->d_iso, sizeof(uint32_t) * iso.size()));
     SFAIL(hipMemcpy(s->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));
@@ -452,6 +457,7 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   a.graphs = s->d_graphs;
   a.iso = s->d_iso;
   a.col_bu = s->d_col_bu;
+  a.head = s->d_head;
   a.req = (ServeRequest *)db;
   a.res = (ServeResult *)(db + sizeof(ServeRequest) * R);
   a.stop = (const uint32_t *)(db + sizeof(ServeRequest) * R + sizeof(ServeResult) * R);

@@ -114,6 +114,7 @@ struct ServeArgs {
   const int64_t *off;           // concatenated row offsets
   const uint32_t *col;          // concatenated columns
   const uint32_t *col_bu;       // the same, rows by neighbour degree (symmetric graphs)
+  const uint4 *head;            // first four col_bu entries per vertex, indexed like off
   const ServeGraph *graphs;
   const uint32_t *iso;          // concatenated isolated-vertex bitmaps (symmetric graphs)
   ServeRequest *req;            // host-mapped ring [ring]
@@ -210,6 +211,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.off = s.off + g.vbase;
     a.col = s.col + g.ebase;
     a.col_bu = g.symmetric ? s.col_bu + g.ebase : nullptr;
+    a.head = g.symmetric ? s.head + g.vbase : nullptr;
     a.level = (uint32_t *)(w + l.level);
     a.parent = parents ? (uint32_t *)(w + l.parent) : nullptr;
     a.iso = g.iso_base != ~0ull ? s.iso + g.iso_base : nullptr;

@@ -156,6 +156,17 @@ inline void order_rows_by_degree(const int64_t *off, const uint32_t *col, uint32
   });
 }
 
+// First four entries of every row of `ordered` (rows by neighbour degree),
+// four words per vertex; words past a row's end are 0 (never tested).
+inline void row_heads(const int64_t *off, const uint32_t *ordered, uint32_t n, uint32_t *out) {
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t v = b; v < e; ++v) {
+      const int64_t s = off[v], t = off[v + 1];
+      for (int i = 0; i < 4; ++i) out[4 * v + i] = s + i < t ? ordered[s + i] : 0u;
+    }
+  });
+}
+
 // Bit v set for every vertex without out-entries.  In a symmetric graph such
 // a vertex has no in-entries either, so no search reaches it: searches may
 // start with these bits already in the visited bitmap, and bottom-up levels

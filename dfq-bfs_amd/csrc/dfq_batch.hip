@@ -301,6 +301,7 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
   for (uint32_t g = 0; g < num_graphs; ++g) {
     const int64_t *off = row_offsets + graphs[g].vbase;
     const uint32_t *col = col_idx + graphs[g].ebase;
+    if (m[g] > kMaxEntries) return DFQ_ERR_RANGE;
     if (off[0] != 0 || (uint64_t)off[n[g]] != m[g]) return DFQ_ERR_RANGE;
     std::atomic<bool> bad{false};
     parallel_for(n[g], [&](uint64_t bb, uint64_t e, unsigned) {

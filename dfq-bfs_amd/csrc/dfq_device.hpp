@@ -103,6 +103,9 @@ __device__ __forceinline__ void tails_prefix(const uint32_t *tails, uint32_t *pr
 constexpr int kMaxXcd = 16;
 constexpr uint64_t kArrive = 1ull << 40;
 constexpr uint64_t kPayloadMask = kArrive - 1;
+// Largest CSR entry count a search accepts: the barrier's payload and side
+// sums (claims, out-degree sums) are 40-bit fields.
+constexpr uint64_t kMaxEntries = kPayloadMask;
 
 struct alignas(256) BarrierWord {
   uint64_t word;

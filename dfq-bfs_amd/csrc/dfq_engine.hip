@@ -288,6 +288,7 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   if (!out || !row_offsets || (m && !col_idx) || n == 0 || n == 0xffffffffu) return DFQ_ERR_INVALID_ARG;
   *out = nullptr;
   // Input checks on the host (the reference trusted its file blindly).
+  if (m > kMaxEntries) return DFQ_ERR_RANGE;
   if (row_offsets[0] != 0 || (uint64_t)row_offsets[n] != m) return DFQ_ERR_RANGE;
   std::atomic<bool> bad{false};
   parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {

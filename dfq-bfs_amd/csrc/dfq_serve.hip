@@ -356,7 +356,7 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   for (uint32_t g = 0; g < num_graphs; ++g) {
     const int64_t *off = row_offsets + graphs[g].vbase;
     const uint32_t *col = col_idx + graphs[g].ebase;
-    if (!csr_valid(off, col, n[g], m[g])) return DFQ_ERR_RANGE;
+    if (m[g] > kMaxEntries || !csr_valid(off, col, n[g], m[g])) return DFQ_ERR_RANGE;
     graphs[g].symmetric = is_symmetric(off, col, n[g]) ? 1u : 0u;
   }
   uint64_t iso_words = 0;

@@ -95,7 +95,8 @@ int dfq_get_device_info(dfq_device_info *info);
 /* Upload a host CSR (row_offsets[n+1] int64, col_idx[m] uint32) to HBM and
  * allocate the BFS working set (bitmap, queues, control block).  Row offsets
  * must be non-decreasing with row_offsets[0] == 0 and row_offsets[n] == m,
- * and every column index < n; violations give DFQ_ERR_RANGE. */
+ * every column index < n, and m < 2^40 (the level loop's grid-wide sums are
+ * 40-bit fields); violations give DFQ_ERR_RANGE. */
 int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx,
                      uint32_t n, uint64_t m, dfq_graph **out);
 void dfq_graph_destroy(dfq_graph *g);

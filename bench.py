@@ -91,6 +91,20 @@ def run_mode(dfq, g, srcs, work, steps, warmup, d_level, launch="seq"):
                 kernel_ms_total=float(t.sum()))
 
 
+def host_cpu():
+    """CPU model and logical core count of the machine running the bench."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(csr, srcs, levels_gpu, budget_s):
     """Serial oracle BFS on as many sources as fit in budget_s (at least 2)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -108,10 +122,33 @@ def cpu_baseline(csr, srcs, levels_gpu, budget_s):
         exact &= bool(np.array_equal(lv, levels_gpu[i]))
         if done >= 2 and secs >= budget_s:
             break
+    model, ncpu = host_cpu()
     return dict(value=edges / secs / 1e9, unit="GTEPS", cores=1, kind="port",
                 sample=f"{done} of the {len(srcs)} sources, serial oracle BFS (C, -O3), "
-                       f"{secs:.1f} s of CPU time on {platform.processor() or platform.machine()}",
+                       f"{secs:.1f} s of CPU time on one core of {model} ({ncpu} logical cores)",
                 levels_bit_exact_on_sample=exact)
+
+
+def other_configs(dfq):
+    """The Amazon-like directed graph (source 0) and the 4096^2 grid with 2%
+    of its edges dropped (source 0): one launch each after a warmup, event
+    time; the grid is barrier-latency bound, so its figure is us per level."""
+    out = {}
+    c1 = dfq.Csr.amazon_like()
+    with dfq.DeviceGraph(c1) as g:
+        g.bfs(0)
+        r = g.bfs(0)
+        e, _, _ = c1.reached_work(r.level)
+        out["amazon_like"] = dict(vertices=c1.n, csr_entries=c1.m, levels=r.depth, ms=round(r.kernel_ms, 4),
+                                  gteps=round(e / (r.kernel_ms * 1e-3) / 1e9, 3))
+    c3 = dfq.Csr.grid2d(4096, 4096, 0.02, 4096)
+    with dfq.DeviceGraph(c3) as g:
+        g.set_config(direction_opt=0)
+        g.bfs(0)
+        r = g.bfs(0)
+        out["grid_4096"] = dict(vertices=c3.n, csr_entries=c3.m, levels=r.depth, ms=round(r.kernel_ms, 3),
+                                us_per_level=round(r.kernel_ms * 1e3 / r.depth, 2))
+    return out
 
 
 def load_traffic(name):
@@ -349,6 +386,7 @@ def main():
                     help="second mode reported beside the headline")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Amazon-like and grid lines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy of BYTES (counter calibration)")
     ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain",
@@ -453,6 +491,8 @@ def main():
         "launch_ms_mean": round(res["kernel_ms_total"], 4),
     }
     out.update(extra)
+    if not args.no_extra:
+        out["other_configs"] = other_configs(dfq)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(csr, srcs, levels, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -90,6 +90,58 @@ def test_closed_forms(dfq):
         assert csr.validate(0, r.level, r.parent)["errors"] == 0
 
 
+@pytest.mark.parametrize("direction_opt", [0, 1])
+def test_huge_hubs_closed_form(dfq, direction_opt):
+    """Undirected hubs at the reference's largest bin edges and beyond:
+    degree 2^20 + 1 (vertex 0, joined to 1..2^20+1), 167,937 (vertex 1),
+    167,936 (vertex 2), 16,385 (vertex 3) and 16,384 (vertex 4), each of
+    the last four with leaves of its own.  Closed form from vertex 0: hub
+    0, its neighbours 1, the other hubs' own leaves 2; from vertex 1: 1 is
+    0, vertex 0 and 1's leaves 1, the rest 2, the other hubs' leaves 3."""
+    big = (1 << 20) + 1
+    degs = [167937, 167936, 16385, 16384]
+    src, dst, lvl0 = [], [], []
+    src.append(np.zeros(big, np.int64))
+    dst.append(np.arange(1, big + 1, dtype=np.int64))
+    nxt = big + 1
+    own = []
+    for h, d in zip([1, 2, 3, 4], degs):
+        leaves = np.arange(nxt, nxt + d, dtype=np.int64)
+        own.append(leaves)
+        src.append(np.full(d, h, np.int64))
+        dst.append(leaves)
+        nxt += d
+    n = nxt
+    s = np.concatenate(src)
+    t = np.concatenate(dst)
+    ro, ci = csr_from_edges_np(n, np.concatenate([s, t]), np.concatenate([t, s]))
+    csr = dfq.Csr.from_arrays(ro, ci)
+    e0 = np.full(n, 1, np.uint32)
+    e0[0] = 0
+    for leaves in own:
+        e0[leaves] = 2
+    e1 = np.full(n, 2, np.uint32)
+    e1[1] = 0
+    e1[0] = 1
+    e1[own[0]] = 1
+    for leaves in own[1:]:
+        e1[leaves] = 3
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(direction_opt=direction_opt)
+        say("huge hubs", direction_opt)
+        r = g.bfs(0, parents=True)
+        np.testing.assert_array_equal(r.level, e0)
+        assert csr.validate(0, r.level, r.parent)["errors"] == 0
+        r = g.bfs(1, parents=True)
+        np.testing.assert_array_equal(r.level, e1)
+        assert csr.validate(1, r.level, r.parent)["errors"] == 0
+        lv = dfq.DeviceBuffer(4 * n * 2)
+        st, _ = g.bfs_device_seq([0, 1], lv.ptr, n)
+        both = lv.to_numpy().reshape(2, n)
+        np.testing.assert_array_equal(both[0], e0)
+        np.testing.assert_array_equal(both[1], e1)
+
+
 def test_single_vertex_and_isolated_source(dfq):
     csr = dfq.Csr.from_arrays([0, 0], np.zeros(0, np.uint32))
     with dfq.DeviceGraph(csr) as g:

@@ -92,7 +92,7 @@ struct BfsArgs {
   uint32_t *parent;      // [n] or null
   const uint32_t *iso;   // isolated vertices [n_words] (symmetric graphs) or null
   uint32_t *vis;         // visited bitmap [n_words]
-  uint32_t *vis_snap;    // its copy taken at each top-down level's start [n_words]
+  uint32_t *vis_snap;    // its copy taken at the start of top-down levels with many out-edges [n_words]
   uint32_t *fb;          // two frontier bitmaps [2][n_words + 1] (bottom-up levels)
   uint32_t *qseg[2];     // frontier queues: kSegments segments of n entries each
   uint2 *items;          // kSegments segments of item_cap work items

@@ -95,6 +95,8 @@ int validate_config(const dfq_config &c) {
   return DFQ_OK;
 }
 
+static_assert(DFQ_PROFILE_FIELDS == kProfFields, "header and kernel disagree on the profile row");
+
 const void *kernel_for(uint32_t block) {
   switch (block) {
     case 1024: return (const void *)&dfq_bfs_kernel<1024>;

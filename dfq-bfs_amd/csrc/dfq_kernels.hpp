@@ -77,7 +77,17 @@ enum ProfField : int {
   kProfLevelNs,       // wall time of the level (workgroup 0, barrier to barrier)
   kProfBusyMaxNs,     // slowest workgroup's expansion time (before its barrier arrival)
   kProfBusySumNs,     // sum over workgroups of the same
-  kProfFields = 10
+  kProfBinSmall,      // frontier vertices per degree bin (levels through the bins):
+  kProfBinMedium,     //   lane-owned, one wave each, one workgroup each,
+  kProfBinLarge,      //   kItem-entry work items over the grid
+  kProfBinHuge,
+  kProfIdleSmall,     // idle lane slots of each bin's last step: 64-lane steps for
+  kProfIdleMedium,    //   the lane-owned, wave and work-item bins, BLOCK-lane steps
+  kProfIdleLarge,     //   for the workgroup bin (the reference's idle-lane counts)
+  kProfIdleHuge,
+  kProfEdgesMaxWg,    // CSR entries read by the busiest workgroup (per-workgroup work)
+  kProfBuCandidates,  // unvisited vertices a bottom-up level scanned
+  kProfFields = 20
 };
 
 // 100 MHz constant-rate counter -> ns
@@ -315,14 +325,42 @@ __device__ __forceinline__ void prof_add(SharedAll<BLOCK> &sh, int f, uint64_t v
   if (lane_id() == 0) atomicAdd((unsigned long long *)&sh.prof[f], (unsigned long long)v);
 }
 
+// Bin counts and idle lane slots of one 64-vertex block of a phase A
+// (profiling only; every lane of the wave calls it).
+template <int BLOCK>
+__device__ __forceinline__ void prof_bins(SharedAll<BLOCK> &sh, bool is_small, bool is_med, bool is_big,
+                                          bool is_huge, uint32_t deg) {
+  const uint32_t ns = (uint32_t)__popcll(__ballot(is_small)), nm = (uint32_t)__popcll(__ballot(is_med));
+  const uint32_t nb = (uint32_t)__popcll(__ballot(is_big)), nh = (uint32_t)__popcll(__ballot(is_huge));
+  uint32_t sdeg = is_small ? deg : 0u;
+  uint32_t im = is_med ? (kWave - deg % kWave) % kWave : 0u;
+  uint32_t ib = is_big ? ((uint32_t)BLOCK - deg % (uint32_t)BLOCK) % (uint32_t)BLOCK : 0u;
+  uint32_t ih = is_huge ? (kWave - deg % kWave) % kWave : 0u;
+  for (int d = 32; d >= 1; d >>= 1) {
+    sdeg += __shfl_xor(sdeg, d);
+    im += __shfl_xor(im, d);
+    ib += __shfl_xor(ib, d);
+    ih += __shfl_xor(ih, d);
+  }
+  if (lane_id() == 0) {
+    // the wave gathers the block's lane-owned edges 64 per step
+    const uint32_t is = (kWave - sdeg % kWave) % kWave;
+    const uint32_t vals[8] = {ns, nm, nb, nh, is, im, ib, ih};
+#pragma unroll
+    for (int f = 0; f < 8; ++f)
+      if (vals[f]) atomicAdd((unsigned long long *)&sh.prof[kProfBinSmall + f], (unsigned long long)vals[f]);
+  }
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &sh, uint32_t L) {
   __syncthreads();
   if (threadIdx.x < kProfFields) {
-    const uint64_t v = sh.prof[threadIdx.x];
+    // the workgroup's own entry count goes into the grid-wide maximum
+    const uint64_t v = threadIdx.x == kProfEdgesMaxWg ? sh.prof[kProfEdges] : sh.prof[threadIdx.x];
     if (v && L < a.prof_levels) {
       unsigned long long *dst = (unsigned long long *)&a.prof[(size_t)L * kProfFields + threadIdx.x];
-      if (threadIdx.x == kProfBusyMaxNs) atomicMax(dst, (unsigned long long)v);
+      if (threadIdx.x == kProfBusyMaxNs || threadIdx.x == kProfEdgesMaxWg) atomicMax(dst, (unsigned long long)v);
       else atomicAdd(dst, (unsigned long long)v);
     }
     sh.prof[threadIdx.x] = 0;
@@ -369,6 +407,7 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
       const bool is_med = deg > a.small_max && deg <= a.medium_max;
       const bool is_big = deg > a.medium_max && deg <= a.large_max;
       const bool is_huge = deg > a.large_max;
+      if (a.prof) prof_bins<BLOCK>(sh, is_small, is_med, is_big, is_huge, deg);
       // wave-aggregated slot reservation (ballot + mbcnt) for the bins
       const uint64_t mm = __ballot(is_med);
       if (mm) {
@@ -589,6 +628,8 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
     const uint32_t cnt = (uint32_t)__popc(unv);
     const uint32_t incl = wave_inclusive_scan(cnt);
     const uint32_t total = bcast_lane(incl, 63);
+    if (a.prof && lane == 0 && total)
+      atomicAdd((unsigned long long *)&sh.prof[kProfBuCandidates], (unsigned long long)total);
     if (lane < kBuWords) cm[lane] = 0;
     for (uint32_t pos = incl - cnt; unv; unv &= unv - 1u) cand[pos++] = (wi << 5) + (uint32_t)(__ffs(unv) - 1);
     wave_sync();

@@ -371,8 +371,11 @@ class DeviceGraph:
     def symmetric(self) -> bool:
         return bool(lib.dfq_graph_is_symmetric(self._h))
 
-    PROFILE_FIELDS = ("frontier", "edges", "probes", "atomics", "claims", "huge", "bottom_up",
-                      "level_ns", "busy_max_ns", "busy_sum_ns")
+    PROFILE_FIELDS = ("frontier", "edges", "probes", "atomics", "claims", "items", "bottom_up",
+                      "level_ns", "busy_max_ns", "busy_sum_ns",
+                      "bin_small", "bin_medium", "bin_large", "bin_huge",
+                      "idle_small", "idle_medium", "idle_large", "idle_huge",
+                      "edges_max_wg", "bu_candidates")
 
     def profile(self, source: int, max_levels: int = 64):
         """Per-level counters: list of dicts (levels beyond max_levels dropped)."""

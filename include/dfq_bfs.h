@@ -151,15 +151,25 @@ int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources,
                        dfq_stats *stats, double *launch_ms);
 
 /* BFS with per-level counters (the reference's frontier profiling build,
- * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,140-144,
- * 200-209).  rows[max_levels][DFQ_PROFILE_FIELDS], one row per level:
+ * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,134-140,
+ * 200-209, and its per-thread workload counts,
+ * design/shared-q/atomic-sa/sup-sep/upper-bound/profile/workload.cuh:8-45).
+ * rows[max_levels][DFQ_PROFILE_FIELDS], one row per level:
  *   0 frontier vertices, 1 CSR entries read, 2 bitmap tests,
- *   3 atomic OR attempts, 4 vertices claimed, 5 grid-expanded vertices,
- *   6 1 if the level ran bottom-up, 7 wall ns of the level,
- *   8 slowest workgroup's expansion ns, 9 sum of the workgroups' ns.
+ *   3 atomic OR attempts, 4 vertices claimed, 5 work items of the grid
+ *   phase (256 CSR entries each), 6 1 if the level ran bottom-up,
+ *   7 wall ns of the level, 8 slowest workgroup's expansion ns,
+ *   9 sum of the workgroups' ns,
+ *   10-13 frontier vertices per degree bin (lane-owned, one wave each,
+ *         one workgroup each, work items) on levels that go through the
+ *         bins (0 on bottom-up levels and edge-balanced small frontiers),
+ *   14-17 idle lane slots of those bins' last steps (64-lane steps;
+ *         workgroup-size steps for the workgroup bin),
+ *   18 CSR entries read by the busiest workgroup,
+ *   19 unvisited vertices scanned by a bottom-up level.
  * Levels beyond max_levels are not recorded.  The counters slow the search;
  * time it with dfq_bfs_device instead. */
-#define DFQ_PROFILE_FIELDS 10
+#define DFQ_PROFILE_FIELDS 20
 int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows,
                     uint32_t max_levels, dfq_stats *stats);
 

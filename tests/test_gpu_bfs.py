@@ -353,3 +353,68 @@ def test_device_seq_bad_input(dfq):
             g.bfs_device_seq([0, csr.n], lv.ptr, csr.n)      # source out of range
         st, ms = g.bfs_device_seq([], lv.ptr)
         assert st == [] and ms == 0.0
+
+
+def test_profile_counters(dfq, oracle):
+    """dfq_bfs_profile on a directed two-level graph whose second frontier
+    (2,000 vertices, more than one per thread, so it goes through the bins)
+    has out-degrees at every bin edge: per-bin counts, the closed-form idle
+    lane slots of the wave, workgroup and work-item bins, entries, claims
+    and frontiers; then level-by-level consistency on RMAT with bottom-up
+    levels."""
+    cfg = dict(block_size=1024, small_max=32, medium_max=256, large_max=4096, direction_opt=0)
+    pattern = [1, 5, 32, 33, 64, 100, 256, 257, 1000, 1024, 4096, 4097, 10000]
+    mids = 2000
+    degs = np.array([pattern[i % len(pattern)] for i in range(mids)], np.int64)
+    src = [np.zeros(mids, np.int64)]
+    dst = [np.arange(1, mids + 1, dtype=np.int64)]
+    nxt = mids + 1
+    for i, d in enumerate(degs):
+        src.append(np.full(d, i + 1, np.int64))
+        dst.append(np.arange(nxt, nxt + d, dtype=np.int64))
+        nxt += d
+    n = nxt
+    ro, ci = csr_from_edges_np(n, np.concatenate(src), np.concatenate(dst))
+    csr = dfq.Csr.from_arrays(ro, ci)
+    small = degs <= 32
+    med = (degs > 32) & (degs <= 256)
+    big = (degs > 256) & (degs <= 4096)
+    huge = degs > 4096
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(**cfg)
+        say("profile bins")
+        rows, st = g.profile(0)
+        assert st.depth == 3 and len(rows) == 3
+        r0, r1 = rows[0], rows[1]
+        assert r0["frontier"] == 1 and r0["claims"] == mids and r0["edges"] == mids
+        assert r1["frontier"] == mids and r1["claims"] == int(degs.sum()) and r1["edges"] == int(degs.sum())
+        assert rows[2]["frontier"] == int(degs.sum()) and rows[2]["claims"] == 0
+        assert (r1["bin_small"], r1["bin_medium"], r1["bin_large"], r1["bin_huge"]) == \
+            (int(small.sum()), int(med.sum()), int(big.sum()), int(huge.sum()))
+        assert r1["idle_medium"] == int(((64 - degs[med] % 64) % 64).sum())
+        assert r1["idle_large"] == int(((1024 - degs[big] % 1024) % 1024).sum())
+        assert r1["idle_huge"] == int(((64 - degs[huge] % 64) % 64).sum())
+        blocks = (mids + 63) // 64
+        assert 0 <= r1["idle_small"] < 64 * blocks
+        assert 0 < r1["edges_max_wg"] <= r1["edges"]
+        assert r1["items"] == int(((degs[huge] + 255) // 256).sum())
+        assert all(r["bottom_up"] == 0 and r["bu_candidates"] == 0 for r in rows)
+        assert sum(r["claims"] for r in rows) == st.reached - 1
+
+    csr = dfq.Csr.rmat(16, 16, seed=9)
+    s = int(csr.pick_sources(1, 3)[0])
+    e = oracle.bfs(csr.row_offsets, csr.col_idx, s)
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(direction_opt=1)
+        say("profile rmat16")
+        rows, st = g.profile(s)
+        assert st.depth == len(rows) == int(e[e != UNREACHED].max()) + 1
+        for L, r in enumerate(rows):
+            assert r["frontier"] == int((e == L).sum())
+            assert r["claims"] == int((e == L + 1).sum())
+            if r["bottom_up"]:
+                assert r["bu_candidates"] > 0
+                assert r["bin_small"] + r["bin_medium"] + r["bin_large"] + r["bin_huge"] == 0
+            else:
+                assert r["bu_candidates"] == 0
+        assert any(r["bottom_up"] for r in rows)

@@ -267,17 +267,24 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
 }
 
 // Moves every wave's staged entries to the segment with one atomic per
-// workgroup.  Must be reached by all threads of the workgroup.
+// workgroup, and sums the wave-uniform `v` over the workgroup (thread 0
+// gets the total in the return value).  Must be reached by all threads of
+// the workgroup.
 template <int BLOCK>
-__device__ __forceinline__ void flush_block(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t wave) {
+__device__ __forceinline__ uint32_t flush_block(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t wave, uint32_t v = 0) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
-  if (lane_id() == 0) sh.stage_n[wave] = vis.out_n;
+  if (lane_id() == 0) {
+    sh.stage_n[wave] = vis.out_n;
+    sh.wave_sum[wave] = v;
+  }
   __syncthreads();
+  uint32_t sum = 0;
   if (threadIdx.x == 0) {
     uint32_t tot = 0;
     for (int i = 0; i < W; ++i) {
       sh.stage_base[i] = tot;
       tot += sh.stage_n[i];
+      sum += sh.wave_sum[i];
     }
     uint32_t base = 0;
     if (tot) base = __hip_atomic_fetch_add(vis.seg_tail, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -288,19 +295,7 @@ __device__ __forceinline__ void flush_block(Visitor &vis, SharedAll<BLOCK> &sh, 
   for (uint32_t k = lane_id(); k < cnt; k += kWave) vis.seg[base + k] = vis.stage[k];
   vis.out_n = 0;
   __syncthreads();
-}
-
-// Sum over the workgroup of a wave-uniform value (thread 0 gets the total in
-// the return value, others get garbage).  Ends with a __syncthreads.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_sum_uniform(SharedAll<BLOCK> &sh, uint32_t v, uint32_t wave) {
-  if (lane_id() == 0) sh.wave_sum[wave] = v;
-  __syncthreads();
-  uint32_t t = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < KernelShape<BLOCK>::kWaves; ++i) t += sh.wave_sum[i];
-  __syncthreads();
-  return t;
+  return sum;
 }
 
 // Workgroup sum of the per-lane out-degree sums of claimed vertices (thread
@@ -997,9 +992,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       else
         topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
                                edges);
-      flush_block<BLOCK>(vis, sh, wave);
+      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       if (a.prof && tid == 0) busy += now_ns() - t_start;
-      const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
       uint64_t mf1 = 0;
       if (solo) {
         n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
@@ -1015,9 +1009,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         uint64_t th = 0;
         if (a.prof && tid == 0) th = now_ns();
         topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
-        flush_block<BLOCK>(vis, sh, wave);
+        const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
         if (a.prof && tid == 0) busy += now_ns() - th;
-        const uint32_t wg2 = block_sum_uniform<BLOCK>(sh, vis.claims - before, wave);
         uint64_t mf2 = 0;
         if (solo)
           n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
@@ -1039,9 +1032,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       uint64_t tb = 0;
       if (a.prof && tid == 0) tb = now_ns();
       bottomup_level<BLOCK>(a, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
-      flush_block<BLOCK>(vis, sh, wave);
+      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       if (a.prof && tid == 0) busy += now_ns() - tb;
-      const uint32_t wg_claims = block_sum_uniform<BLOCK>(sh, vis.claims, wave);
       uint64_t mf1 = 0;
       n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
       m_f = mf1;

@@ -50,10 +50,10 @@ def log(*a):
 MODES = {
     # every level top-down through the degree-binned frontier queues
     "topdown": dict(block_size=1024, small_max=32, medium_max=256, large_max=256,
-                    direction_opt=0, bu_enter_div=40, bu_exit_div=24),
+                    direction_opt=0, bu_enter_div=40, bu_exit_div=96),
     # the same kernel with bottom-up levels for large frontiers
     "hybrid": dict(block_size=1024, small_max=32, medium_max=256, large_max=256,
-                   direction_opt=1, bu_enter_div=40, bu_exit_div=24),
+                   direction_opt=1, bu_enter_div=40, bu_exit_div=96),
 }
 
 

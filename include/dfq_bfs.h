@@ -49,7 +49,8 @@ extern "C" {
  * default 40, measured on MI355X, where a top-down level costs relatively
  * more than on the CPUs alpha = 14 was tuned for), and returns to top-down
  * when a shrinking frontier holds fewer
- * than n / bu_exit_div vertices (beta, default 24).  It only takes effect on
+ * than n / bu_exit_div vertices (beta, default 96, measured on MI355X:
+ * bottom-up levels over few remaining vertices are cheap there).  It only takes effect on
  * symmetric graphs (checked when the graph is created); levels are the same
  * either way.  direction_opt = 0 runs every level top-down. */
 typedef struct dfq_config {

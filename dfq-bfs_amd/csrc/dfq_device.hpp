@@ -24,6 +24,9 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// 100 MHz constant-rate counter -> ns
+__device__ __forceinline__ uint64_t now_ns() { return __builtin_amdgcn_s_memrealtime() * 10ull; }
+
 // Orders LDS traffic between the lanes of one wavefront.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -122,6 +125,10 @@ struct GridBarrier {
   BarrierWord census_epoch;     // launches counted so far (low bit picks the census set)
   BarrierWord ticket;           // workgroup ranks handed out within a team
   uint32_t census[2][kMaxXcd];  // workgroups per XCD, one set per launch parity
+  // plain (non-cooperative) launches, by the parity of the host's launch
+  // number: arrivals (low bits) | kPlainAbandon, and workgroups per XCD
+  BarrierWord plain_word[2];
+  uint32_t plain_census[2][kMaxXcd];
 };
 
 // Barrier state of one workgroup within its team (thread 0's copy counts).
@@ -214,6 +221,83 @@ __device__ __forceinline__ void barrier_census(GridBarrier *b, TeamBar team_bar,
     for (int i = 0; i < kMaxXcd; ++i) __hip_atomic_store(&other[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&b->census_epoch.word, epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Kernel start of a plain (non-cooperative) launch, wave 0 of every
+// workgroup.  Nothing guarantees that every workgroup of such a launch is
+// resident at once (other work may hold CUs), so the grid assembles itself
+// with a bounded wait and one decision word: every workgroup adds itself to
+// its XCD's counter and then to the arrival count; it proceeds once the
+// count reaches the grid with the abandon bit clear, and gives up when the
+// bit is set.  A workgroup that has waited `wait_ns` sets the bit with a
+// compare-and-swap that only succeeds while the count is short, so a grid
+// that completed can no longer be abandoned and an abandoned one never
+// proceeds: all workgroups decide alike, late ones included (they see the
+// bit in their own arrival's result).  The host then relaunches
+// cooperatively.  `epoch` is the host's number of the launch: alternate
+// launches use alternate words, and the workgroup that decides (workgroup
+// 0 on success, the one whose swap succeeded otherwise) clears the other
+// parity's words for the next launch, whose predecessor has ended.
+// Returns false in every workgroup when the launch is abandoned; otherwise
+// fills ci, count and gen as barrier_census does.
+constexpr uint64_t kPlainAbandon = 1ull << 63;
+template <class TeamBar>
+__device__ __forceinline__ bool plain_census(GridBarrier *b, uint32_t epoch, uint64_t wait_ns, TeamBar team_bar,
+                                             CensusInfo &ci, uint32_t *count, uint32_t *gen) {
+  const uint32_t lane = lane_id();
+  const uint32_t p = epoch & 1u;
+  ci.xcc = xcc_id();
+  uint32_t *cnt = b->plain_census[p];
+  uint64_t *word = &b->plain_word[p].word;
+  uint32_t go = 0;
+  if (lane == 0) {
+    __hip_atomic_fetch_add(&cnt[ci.xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_own_stores();  // the per-XCD count is in before the arrival counts
+    const uint64_t old = __hip_atomic_fetch_add(word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(old & kPlainAbandon)) {
+      const uint64_t t0 = now_ns();
+      uint64_t w = ld_relaxed(word);
+      for (;;) {
+        if (w & kPlainAbandon) break;
+        if ((w & ~kPlainAbandon) >= gridDim.x) {
+          go = 1;
+          break;
+        }
+        if (now_ns() - t0 >= wait_ns) {
+          if (__hip_atomic_compare_exchange_strong(word, &w, w | kPlainAbandon, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            // this workgroup abandoned the launch: ready the next one's words
+            uint32_t *oc = b->plain_census[p ^ 1u];
+            for (int i = 0; i < kMaxXcd; ++i)
+              __hip_atomic_store(&oc[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&b->plain_word[p ^ 1u].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          continue;  // w now holds the word's current value
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w = ld_relaxed(word);
+      }
+    }
+  }
+  go = (uint32_t)__shfl((int)go, 0);
+  if (!go) return false;
+  // the grid is complete, so the per-XCD counts are final
+  uint32_t c = 0, g = 0;
+  if (lane < (uint32_t)kMaxXcd) c = ld_relaxed(&cnt[lane]);
+  else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> 40);
+  const uint32_t nz = (uint32_t)__ballot(lane < (uint32_t)kMaxXcd && c != 0);
+  const uint32_t below = nz & ((1u << (lane & 31u)) - 1u);
+  if (lane < (uint32_t)kMaxXcd && c != 0) count[__popc(below)] = c;
+  if (lane >= (uint32_t)kMaxXcd && lane < 2u * kMaxXcd) gen[lane - kMaxXcd] = g;
+  ci.nx = __popc(nz);
+  ci.xrank = __popc(nz & ((1u << ci.xcc) - 1u));
+  if (blockIdx.x == 0 && lane == 0) {
+    uint32_t *oc = b->plain_census[p ^ 1u];
+    for (int i = 0; i < kMaxXcd; ++i) __hip_atomic_store(&oc[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&b->plain_word[p ^ 1u].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
 }
 
 // Team of XCD ranks [id * xpt, id * xpt + xpt) (thread 0).  `tb` is the

@@ -49,6 +49,7 @@ hipError_t &last_hip_error() {
 
 struct dfq_graph {
   int device = -1;
+  uint32_t launches = 0;  // number of the last plain launch (never 0; parity alternates)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t n = 0;
@@ -91,7 +92,7 @@ int validate_config(const dfq_config &c) {
   if (c.medium_max < c.small_max || c.large_max < c.medium_max) return DFQ_ERR_INVALID_ARG;
   if (c.direction_opt > 1) return DFQ_ERR_INVALID_ARG;
   if (c.direction_opt && (c.bu_enter_div == 0 || c.bu_exit_div == 0)) return DFQ_ERR_INVALID_ARG;
-  if (c.reserved != 0) return DFQ_ERR_INVALID_ARG;
+  if (c.launch_mode > DFQ_LAUNCH_PLAIN_ABANDON) return DFQ_ERR_INVALID_ARG;
   return DFQ_OK;
 }
 
@@ -218,19 +219,57 @@ void fill_stats(const dfq_graph *g, const uint64_t *r, double ms, int grid, dfq_
   st.block_size = (uint32_t)g->block;
 }
 
+// How long a workgroup of a plain launch waits for the rest of the grid
+// before the launch is given up (a resident grid assembles in ~2 us).
+constexpr uint64_t kPlainWaitNs = 1000000;
+
+// Number of the next plain launch: never 0, and consecutive launches
+// alternate in parity (the kernel's census words go by parity).
+uint32_t next_launch(dfq_graph *g) {
+  g->launches = g->launches == 0xffffffffu ? 2u : g->launches + 1u;
+  return g->launches;
+}
+
+// Sets up `a` for a plain launch (numbered) or a cooperative one.
+void launch_kind(dfq_graph *g, BfsArgs &a, bool plain) {
+  a.plain = plain ? next_launch(g) : 0u;
+  a.plain_wait_ns = g->cfg.launch_mode == DFQ_LAUNCH_PLAIN_ABANDON ? 0u : kPlainWaitNs;
+}
+
+// One launch of the single-search kernel on the graph's stream.  A plain
+// launch costs ~1.5 us of dispatch between kernels where a cooperative
+// one costs ~19 us on this stack; the kernel makes a plain launch safe by
+// assembling its grid itself (and giving it up when it cannot).
+hipError_t launch_search(dfq_graph *g, BfsArgs &a) {
+  void *args[] = {&a};
+  if (a.plain)
+    return hipLaunchKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0, g->stream);
+  return hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0,
+                                    g->stream);
+}
+
 int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats,
             bool prof = false) {
   if (source >= g->n) return DFQ_ERR_RANGE;
   HIP_TRY(hipSetDevice(g->device));
   BfsArgs a = make_args(g, source, d_level, d_parent, prof);
-  void *args[] = {&a};
+  launch_kind(g, a, g->cfg.launch_mode != DFQ_LAUNCH_COOPERATIVE);
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block),
-                                     args, 0, g->stream));
+  HIP_TRY(launch_search(g, a));
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipEventSynchronize(g->ev1));
   Control c;
   HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  if (a.plain && c.abandoned == a.plain) {
+    // the grid did not assemble (other work held CUs): run cooperatively
+    if (prof) HIP_TRY(hipMemset(g->d_prof, 0, sizeof(uint64_t) * kProfFields * g->prof_levels));
+    launch_kind(g, a, false);
+    HIP_TRY(hipEventRecord(g->ev0, g->stream));
+    HIP_TRY(launch_search(g, a));
+    HIP_TRY(hipEventRecord(g->ev1, g->stream));
+    HIP_TRY(hipEventSynchronize(g->ev1));
+    HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  }
   if (c.err) return DFQ_ERR_CAPACITY;
   if (stats) {
     float ms = 0.f;
@@ -453,23 +492,33 @@ int dfq_bfs_device_many(dfq_graph *g, uint32_t count, const uint32_t *sources, u
     g->evs.push_back(e);
   }
   HIP_TRY(reserve_results(g, count));
-  // all launches are queued before the host waits, so each event pair
-  // brackets one kernel and not the host's submission of it
-  HIP_TRY(hipEventRecord(g->evs[0], g->stream));
-  for (uint32_t i = 0; i < count; ++i) {
-    BfsArgs a = make_args(g, sources[i], d_level, d_parent, false);
-    a.result = g->d_res + (size_t)kResultWords * i;
-    void *args[] = {&a};
-    HIP_TRY(hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0,
-                                       g->stream));
-    HIP_TRY(hipEventRecord(g->evs[i + 1], g->stream));
-  }
-  HIP_TRY(hipEventSynchronize(g->evs[count]));
-  uint32_t err = 0;
-  HIP_TRY(hipMemcpy(&err, &g->d_ctl->err, sizeof(err), hipMemcpyDeviceToHost));
-  if (err) return DFQ_ERR_CAPACITY;
   std::vector<uint64_t> res((size_t)kResultWords * count);
-  HIP_TRY(hipMemcpy(res.data(), g->d_res, sizeof(uint64_t) * res.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> num(count);
+  bool plain = g->cfg.launch_mode != DFQ_LAUNCH_COOPERATIVE;
+  for (;;) {
+    // all launches are queued before the host waits, so each event pair
+    // brackets one kernel and not the host's submission of it
+    HIP_TRY(hipEventRecord(g->evs[0], g->stream));
+    for (uint32_t i = 0; i < count; ++i) {
+      BfsArgs a = make_args(g, sources[i], d_level, d_parent, false);
+      a.result = g->d_res + (size_t)kResultWords * i;
+      launch_kind(g, a, plain);
+      num[i] = a.plain;
+      HIP_TRY(launch_search(g, a));
+      HIP_TRY(hipEventRecord(g->evs[i + 1], g->stream));
+    }
+    HIP_TRY(hipEventSynchronize(g->evs[count]));
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, &g->d_ctl->err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) return DFQ_ERR_CAPACITY;
+    HIP_TRY(hipMemcpy(res.data(), g->d_res, sizeof(uint64_t) * res.size(), hipMemcpyDeviceToHost));
+    bool abandoned = false;
+    for (uint32_t i = 0; i < count && plain; ++i) abandoned |= res[(size_t)kResultWords * i + 6] == num[i];
+    if (!abandoned) break;
+    // a grid did not assemble: the whole sequence again, cooperatively, so
+    // the output still ends with the last source's search
+    plain = false;
+  }
   for (uint32_t i = 0; i < count; ++i) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, g->evs[i], g->evs[i + 1]));

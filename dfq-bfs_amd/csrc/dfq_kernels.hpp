@@ -58,7 +58,7 @@ struct Control {
   uint32_t err;       // nonzero: a queue capacity was exceeded
   uint64_t reached;   // vertices with a level
   uint32_t bu_levels; // levels run bottom-up
-  uint32_t pad0;
+  uint32_t abandoned; // number of the last plain launch whose grid did not assemble
   uint64_t edges;     // CSR entries of reached vertices (when counted)
   uint32_t pad[56];
   BarrierWord solo_rel;        // tag of the last hand-back of solo levels
@@ -90,8 +90,6 @@ enum ProfField : int {
   kProfFields = 20
 };
 
-// 100 MHz constant-rate counter -> ns
-__device__ __forceinline__ uint64_t now_ns() { return __builtin_amdgcn_s_memrealtime() * 10ull; }
 
 struct BfsArgs {
   const int64_t *off;    // row offsets [n+1]
@@ -110,7 +108,10 @@ struct BfsArgs {
   Control *ctl;
   uint64_t *prof;        // [prof_levels][kProfFields] or null
   uint64_t *result;      // [kResultWords] or null: depth, reached, bottom-up levels, edges,
-                         // device-clock ns at the search's start and end
+                         // device-clock ns at the search's start and end, the launch
+                         // number when a plain launch was abandoned
+  uint32_t plain;        // 0: cooperative launch; else the host's number of this plain launch
+  uint64_t plain_wait_ns; // plain launches: how long a workgroup waits for the grid to assemble
   uint32_t prof_levels;
   uint32_t n;
   uint32_t n_words;
@@ -1084,11 +1085,27 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
 // Kernel start: census of workgroups per XCD and formation of teams of
 // `xpt` XCDs each, whose barriers are team_bar(id).  Returns false in every
 // workgroup alike when a team would span more XCDs than frontier segments.
+// plain: 0 for a cooperative launch, else the host's number of this plain
+// launch, whose grid assembles itself (plain_census); when it does not,
+// every workgroup returns false with *abandoned set.
 template <int BLOCK, class TeamBar>
 __device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_t xpt, bool by_ticket, TeamBar team_bar,
-                                SharedAll<BLOCK> &sh, Team &team) {
+                                SharedAll<BLOCK> &sh, Team &team, uint32_t plain = 0, uint64_t plain_wait_ns = 0,
+                                bool *abandoned = nullptr) {
   CensusInfo ci;
-  if (threadIdx.x < kWave) barrier_census(census_bar, team_bar, ci, sh.census, sh.census + kMaxXcd);
+  if (plain) {
+    if (threadIdx.x < kWave) {
+      const bool go = plain_census(census_bar, plain, plain_wait_ns, team_bar, ci, sh.census, sh.census + kMaxXcd);
+      if (threadIdx.x == 0) sh.bcast[1] = go ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!sh.bcast[1]) {
+      *abandoned = true;
+      return false;
+    }
+  } else if (threadIdx.x < kWave) {
+    barrier_census(census_bar, team_bar, ci, sh.census, sh.census + kMaxXcd);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     if (xpt == 0 || xpt > ci.nx) xpt = ci.nx;
@@ -1115,8 +1132,18 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   uint64_t solo_seen = 0;
   if (threadIdx.x == 0) solo_seen = ld_relaxed(&a.ctl->solo_rel.word);
   bfs_init(a, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
-  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->err = 2;
+  bool abandoned = false;
+  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team, a.plain, a.plain_wait_ns,
+                              &abandoned)) {
+    if (abandoned) {
+      // a plain launch whose grid did not assemble: the host reruns it
+      if (threadIdx.x == 0) {
+        a.ctl->abandoned = a.plain;
+        if (a.result) a.result[6] = a.plain;
+      }
+    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.ctl->err = 2;
+    }
     return;
   }
   grid_barrier(bar, team.bst, 0, sh.bcast);

@@ -55,6 +55,8 @@ if not os.path.exists(LIB_PATH):
 
 lib = C.CDLL(LIB_PATH)
 
+LAUNCH_PLAIN, LAUNCH_COOPERATIVE, LAUNCH_PLAIN_ABANDON = 0, 1, 2
+
 
 class DfqError(RuntimeError):
     def __init__(self, code: int, what: str):
@@ -72,10 +74,10 @@ class Config(C.Structure):
                 ("small_max", C.c_uint32), ("medium_max", C.c_uint32),
                 ("large_max", C.c_uint32), ("direction_opt", C.c_uint32),
                 ("bu_enter_div", C.c_uint32), ("bu_exit_div", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("launch_mode", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Stats(C.Structure):

@@ -62,11 +62,25 @@ typedef struct dfq_config {
   uint32_t direction_opt;  /* 0 or 1                                       */
   uint32_t bu_enter_div;   /* alpha, > 0 when direction_opt                */
   uint32_t bu_exit_div;    /* beta, > 0 when direction_opt                 */
-  uint32_t reserved;       /* must be zero                                 */
+  uint32_t launch_mode;    /* DFQ_LAUNCH_*, default DFQ_LAUNCH_PLAIN        */
 } dfq_config;
 
+/* How dfq_bfs / dfq_bfs_device / dfq_bfs_device_many / dfq_bfs_profile
+ * launch the search kernel.  The reference launched cooperatively
+ * (bfs.cu:283); on this stack a cooperative launch costs about 19 us
+ * between kernels against 1.5 us for a plain one, so by default the kernel
+ * is launched plainly and assembles its grid itself: its workgroups count
+ * themselves in with a bounded wait, and if the whole grid is not resident
+ * within 1 ms (other work holding CUs) every workgroup gives up alike and
+ * the search is run again with a cooperative launch.  Results are the same
+ * either way.  dfq_bfs_device_seq always launches cooperatively (once per
+ * sequence). */
+#define DFQ_LAUNCH_PLAIN 0         /* plain launch, cooperative when it gives up */
+#define DFQ_LAUNCH_COOPERATIVE 1   /* always cooperative                         */
+#define DFQ_LAUNCH_PLAIN_ABANDON 2 /* plain launch that gives up at once (tests the fallback) */
+
 typedef struct dfq_stats {
-  double   kernel_ms;        /* hipEvent time of the cooperative launch   */
+  double   kernel_ms;        /* hipEvent time of the search's launch      */
   uint32_t depth;            /* number of BFS levels (max level + 1)      */
   uint32_t grid_blocks;      /* workgroups launched                       */
   uint32_t block_size;       /* threads per workgroup                     */

@@ -122,3 +122,15 @@ def test_server_create_rejects_bad_input(dfq):
     assert dfq.lib.dfq_server_start(None) == dfq.ERR_INVALID_ARG
     assert dfq.lib.dfq_server_release(None, 0) == dfq.ERR_INVALID_ARG
     assert dfq.lib.dfq_strerror(dfq.ERR_PENDING) == b"query still running"
+
+
+def test_config_default_and_launch_modes(dfq):
+    """Defaults need no GPU; the launch-mode constants of the header and the
+    binding agree."""
+    c = dfq.Config()
+    assert dfq.lib.dfq_config_default(dfq.C.byref(c)) == 0
+    assert c.launch_mode == dfq.LAUNCH_PLAIN and c.bu_enter_div == 40 and c.bu_exit_div == 96
+    text = open(os.path.join(ROOT, "include", "dfq_bfs.h")).read()
+    consts = dict(re.findall(r"#define (DFQ_LAUNCH_[A-Z_]+) (\d+)", text))
+    assert consts == {"DFQ_LAUNCH_PLAIN": "0", "DFQ_LAUNCH_COOPERATIVE": "1", "DFQ_LAUNCH_PLAIN_ABANDON": "2"}
+    assert (dfq.LAUNCH_PLAIN, dfq.LAUNCH_COOPERATIVE, dfq.LAUNCH_PLAIN_ABANDON) == (0, 1, 2)

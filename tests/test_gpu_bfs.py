@@ -418,3 +418,43 @@ def test_profile_counters(dfq, oracle):
             else:
                 assert r["bu_candidates"] == 0
         assert any(r["bottom_up"] for r in rows)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_launch_modes(dfq, oracle, mode):
+    """Plain launches that assemble their grid (mode 0, the default),
+    cooperative launches (1), and plain launches that give up at once and
+    are run again cooperatively (2): the same levels and parents for single
+    searches, back-to-back searches and the profiled search, and searches
+    after a given-up launch are unaffected."""
+    csr = dfq.Csr.rmat(14, 16, seed=31)
+    srcs = [int(s) for s in csr.pick_sources(6, 7)]
+    expect = {s: oracle.bfs(csr.row_offsets, csr.col_idx, s) for s in srcs}
+    lv = dfq.DeviceBuffer(4 * csr.n)
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(direction_opt=1, launch_mode=mode)
+        say("launch mode", mode)
+        for s in srcs:
+            r = g.bfs(s, parents=True)
+            np.testing.assert_array_equal(r.level, expect[s])
+            assert csr.validate(s, r.level, r.parent)["errors"] == 0
+        stats = g.bfs_device_many(srcs, lv.ptr)
+        for s, st in zip(srcs, stats):
+            assert st.reached == int((expect[s] != UNREACHED).sum())
+        np.testing.assert_array_equal(lv.to_numpy(), expect[srcs[-1]])
+        rows, st = g.profile(srcs[0])
+        assert sum(r["claims"] for r in rows) == st.reached - 1
+        # alternate modes on the same graph: census words of every kind stay consistent
+        g.set_config(launch_mode=(mode + 1) % 3)
+        np.testing.assert_array_equal(g.bfs(srcs[1]).level, expect[srcs[1]])
+        g.set_config(launch_mode=mode)
+        np.testing.assert_array_equal(g.bfs(srcs[2]).level, expect[srcs[2]])
+        st2, _ = g.bfs_device_seq(srcs, lv.ptr)
+        np.testing.assert_array_equal(lv.to_numpy(), expect[srcs[-1]])
+
+
+def test_launch_mode_rejected(dfq):
+    csr = dfq.Csr.rmat(8, 4, seed=1)
+    with dfq.DeviceGraph(csr) as g:
+        with pytest.raises(dfq.DfqError):
+            g.set_config(launch_mode=3)

@@ -20,8 +20,9 @@ prices the algorithmic bytes of the step, the sum over the searches of
 B_alg = 4 E_r + 16 V_r + 4 N (CSR entries, two int64 offsets per reached
 vertex, the level array written in full), against 8 TB/s over the event
 time of the launch.  "one_launch_per_search" repeats the measurement with
-one cooperative launch per source (dfq_bfs_device_many, each launch
-bracketed by its own hipEvents), the reference's way of running a search.
+one launch per source (dfq_bfs_device_many: plain launches whose grid
+assembles itself, each bracketed by its own hipEvents), the reference's
+way of running a search.
 
 cpu_baseline times the oracle (serial CPU BFS restating the reference's
 host loop) on a bounded sample of the same sources and also checks that the

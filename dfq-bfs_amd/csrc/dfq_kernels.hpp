@@ -803,30 +803,38 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
 // device), `gtid` of `gstride` threads: levels, visited bitmap and its
 // snapshot (isolated vertices premarked), frontier bitmaps, and the
 // level-0 frontier (the source alone, in segment 0 of buffer 0, whose
-// tails live in the slot of "level -1", slot 3).
+// tails live in the slot of "level -1", slot 3).  kThrough: every store is
+// an agent-scope one, written through this XCD's L2, so once this thread
+// has waited for its stores they are visible to every XCD without an L2
+// write-back.
+template <bool kThrough = false>
 __device__ __forceinline__ void bfs_init(const BfsArgs &a, uint32_t gtid, uint32_t gstride) {
+  auto put = [](uint32_t *p, uint32_t x) {
+    if (kThrough) __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = x;
+  };
   for (uint32_t i = gtid; i < a.n; i += gstride) {
-    a.level[i] = (i == a.source) ? 0u : kInf;
-    if (a.parent) a.parent[i] = (i == a.source) ? a.source : kInf;
+    put(&a.level[i], (i == a.source) ? 0u : kInf);
+    if (a.parent) put(&a.parent[i], (i == a.source) ? a.source : kInf);
   }
   for (uint32_t w = gtid; w < a.n_words; w += gstride) {
     const uint32_t init = ((w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u) | (a.iso ? a.iso[w] : 0u);
-    a.vis[w] = init;
-    a.vis_snap[w] = init;
+    put(&a.vis[w], init);
+    put(&a.vis_snap[w], init);
   }
   const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
   if (a.bottom_up)
-    for (size_t w = gtid; w < 2 * fbs; w += gstride) a.fb[w] = 0;
+    for (size_t w = gtid; w < 2 * fbs; w += gstride) put(&a.fb[w], 0u);
   if (gtid == 0) {
     Control *ctl = a.ctl;
-    a.qseg[0][0] = a.source;
+    put(&a.qseg[0][0], a.source);
     for (int s = 0; s < 4; ++s)
       for (int k = 0; k < kSegments; ++k) {
-        ctl->slot[s].tail[k] = 0;
-        ctl->slot[s].item_tail[k] = 0;
+        put(&ctl->slot[s].tail[k], 0u);
+        put(&ctl->slot[s].item_tail[k], 0u);
       }
-    ctl->slot[3].tail[0] = 1;
-    ctl->err = 0;
+    put(&ctl->slot[3].tail[0], 1u);
+    put(&ctl->err, 0u);
   }
 }
 
@@ -1125,13 +1133,17 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
   GridBarrier *bar = &a.ctl->bar;
-  // One search per launch: its initial state is written while the census
-  // gathers the grid, each workgroup its blockIdx share, and released by
-  // the search's first grid barrier.  solo_rel is read before this
-  // workgroup is counted, so before any hand-back of this search.
+  // One search per launch: each workgroup writes its blockIdx share of the
+  // initial state with stores that go through to memory and waits for
+  // them before it is counted, so the census, which every workgroup
+  // passes only after all were counted, releases the state (no grid
+  // barrier of its own).  solo_rel is read before this workgroup is
+  // counted, so before any hand-back of this search.
   uint64_t solo_seen = 0;
   if (threadIdx.x == 0) solo_seen = ld_relaxed(&a.ctl->solo_rel.word);
-  bfs_init(a, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
+  bfs_init<true>(a, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
+  wait_own_stores();
+  __syncthreads();
   bool abandoned = false;
   if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh, team, a.plain, a.plain_wait_ns,
                               &abandoned)) {
@@ -1146,7 +1158,9 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
     }
     return;
   }
-  grid_barrier(bar, team.bst, 0, sh.bcast);
+  // acquire: drop lines this CU or XCD may hold from before the release
+  if (threadIdx.x == 0) l2_invalidate();
+  __syncthreads();
   // the kernel ends after the search, which writes every L2 back
   grid_bfs<BLOCK>(a, sh, team, GridBfsMode{true, true, solo_seen});
 }

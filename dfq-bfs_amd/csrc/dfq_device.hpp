@@ -443,6 +443,7 @@ struct Visitor {
   uint32_t probes, atomics;  // profiling counters (per lane)
   const int64_t *off;        // row offsets, read for claimed vertices when track_mf
   bool track_mf;
+  bool deg32;                // every degree < 2^32: the low words of two offsets give it
   uint64_t mf;               // out-degree sum of vertices claimed by this lane
   uint32_t *fb_mark;         // when set, claims also set their bit here (next frontier bitmap)
 
@@ -479,7 +480,12 @@ struct Visitor {
   // then the atomic.  On levels without a snapshot `snap` is the live
   // bitmap and is probed once.  The loads can only under-report visited
   // vertices, never over-report.
-  template <int U>
+  //
+  // kEarlyDeg (with deg32): the low words of the row offsets of the
+  // vertices tried (for the out-degree sum) are loaded with the claim
+  // attempts, so one wait covers both; otherwise the offsets are loaded
+  // for the claimed vertices only, after the attempts return.
+  template <int U, bool kEarlyDeg>
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
                                               const uint32_t (&w)[U]) {
     uint32_t word[U];
@@ -490,14 +496,21 @@ struct Visitor {
       for (int i = 0; i < U; ++i)
         if (!((word[i] >> (w[i] & 31u)) & 1u)) word[i] = vis[w[i] >> 5];
     }
-    uint32_t old[U];
+    const bool early = kEarlyDeg && track_mf && deg32;
+    uint32_t old[U], lo0[U], lo1[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) {
       const uint32_t bit = 1u << (w[i] & 31u);
       old[i] = bit;
+      lo0[i] = lo1[i] = 0;
       if (!(word[i] & bit)) {
         ++atomics;
         old[i] = __hip_atomic_fetch_or(&vis[w[i] >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (early) {
+          const uint32_t *o = reinterpret_cast<const uint32_t *>(off + w[i]);
+          lo0[i] = o[0];
+          lo1[i] = o[2];
+        }
       }
     }
 #pragma unroll
@@ -509,7 +522,8 @@ struct Visitor {
         if (claimed) {
           level[w[i]] = next_level;
           if (parent) parent[w[i]] = u[i];
-          if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+          if (early) mf += (uint64_t)(lo1[i] - lo0[i]);
+          else if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
           if (fb_mark)
             __hip_atomic_fetch_or(&fb_mark[w[i] >> 5], 1u << (w[i] & 31u), __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);

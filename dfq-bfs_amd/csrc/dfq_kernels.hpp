@@ -215,7 +215,7 @@ __device__ __forceinline__ void gather_small(Visitor &vis, uint32_t v, int64_t s
       u[i] = g_v[j];
       w[i] = act[i] ? vis.col[g_s[j] + (int64_t)(e - g_excl[j])] : 0u;
     }
-    vis.visit_batch<kUnroll>(act, u, w);
+    vis.visit_batch<kUnroll, true>(act, u, w);
   }
   wave_sync();
 }
@@ -240,7 +240,7 @@ __device__ __forceinline__ void expand_medium(Visitor &vis, SharedAll<BLOCK> &sh
         uu[i] = u;
         w[i] = act[i] ? vis.col[s + k] : 0u;
       }
-      vis.visit_batch<kUnroll>(act, uu, w);
+      vis.visit_batch<kUnroll, true>(act, uu, w);
     }
   }
 }
@@ -262,7 +262,7 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
         uu[i] = u;
         w[i] = act[i] ? vis.col[s + k] : 0u;
       }
-      vis.visit_batch<kUnroll>(act, uu, w);
+      vis.visit_batch<kUnroll, true>(act, uu, w);
     }
   }
 }
@@ -533,7 +533,7 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
       w[i] = act[i] ? a.col[sh.u.f.start[j] + (int64_t)(e - sh.u.f.pre[j])] : 0u;
       if (act[i]) ++edges;
     }
-    vis.visit_batch<kUnroll>(act, u, w);
+    vis.visit_batch<kUnroll, false>(act, u, w);
   }
   __syncthreads();
 }
@@ -563,7 +563,7 @@ __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOC
       uu[k] = u;
       w[k] = act[k] ? a.col[s + e] : 0u;
     }
-    vis.visit_batch<kUnroll>(act, uu, w);
+    vis.visit_batch<kUnroll, false>(act, uu, w);
   }
 }
 
@@ -897,6 +897,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   // out-degree sums of the claimed vertices: the direction heuristic, the
   // choice of the small-frontier expansion, and ctl->edges
   vis.track_mf = true;
+  vis.deg32 = a.m <= 0xffffffffull;  // no row can have 2^32 entries
   vis.mf = 0;
 
   uint64_t n_cur = 1, n_prev = 0, reached = 1;

@@ -100,6 +100,7 @@ struct WgVisitor {
   uint32_t *out_n;     // LDS
   uint32_t next_level;
   bool track_mf;
+  bool deg32;  // every degree < 2^32: the low words of two offsets give it
   uint64_t mf;
 
   __device__ __forceinline__ void append(bool mine, uint64_t mask, uint32_t w) {
@@ -109,22 +110,38 @@ struct WgVisitor {
     if (mine) q_out[base + mbcnt(mask)] = w;
   }
 
+  // Claims in LDS; with deg32 the claimed vertices' row offsets (low
+  // words) are summed after the whole batch, so their loads are waited for
+  // once per batch instead of once per claim.
   template <int U>
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&w)[U]) {
+    const bool late = track_mf && deg32;
+    uint32_t lo0[U], lo1[U];
 #pragma unroll
     for (int i = 0; i < U; ++i) {
       bool claimed = false;
+      lo0[i] = lo1[i] = 0;
       if (act[i]) {
         const uint32_t bit = 1u << (w[i] & 31u);
         if (!(vis[w[i] >> 5] & bit)) claimed = !(atomicOr(&vis[w[i] >> 5], bit) & bit);
         if (claimed) {
           level[w[i]] = next_level;
           atomicOr(&fb_next[w[i] >> 5], bit);
-          if (track_mf) mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+          if (late) {
+            const uint32_t *o = reinterpret_cast<const uint32_t *>(off + w[i]);
+            lo0[i] = o[0];
+            lo1[i] = o[2];
+          } else if (track_mf) {
+            mf += (uint64_t)(off[w[i] + 1] - off[w[i]]);
+          }
         }
       }
       const uint64_t mask = __ballot(claimed);
       if (mask) append(claimed, mask, w[i]);
+    }
+    if (late) {
+#pragma unroll
+      for (int i = 0; i < U; ++i) mf += (uint64_t)(lo1[i] - lo0[i]);  // 0 where nothing was claimed
     }
   }
 };
@@ -214,6 +231,7 @@ __device__ __forceinline__ void wg_bfs(SharedWg<BLOCK> &sh, const int64_t *off, 
   vis.vis = sh.vis;
   vis.out_n = &sh.out_n;
   vis.track_mf = q.direction_opt && symmetric;
+  vis.deg32 = (uint64_t)(off[n] - off[0]) <= 0xffffffffull;  // no row can have 2^32 entries
 
   uint32_t n_cur = 1, n_prev = 0, bu_levels = 0;
   uint64_t reached = 1;

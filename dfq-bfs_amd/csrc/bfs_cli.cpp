@@ -11,8 +11,9 @@
 // list (.txt/.snap/.el; "# Nodes: N Edges: M" header, README.md:19-32) or a
 // binary CSR cache (.csr).  The report keeps the reference's lines
 // "# Nodes :", "# Edges :", "Source  :", "- Dist  =", "- Touch =",
-// "- Error =", "- Time  = <ms>ms" (:206-208, :297-300) so harnesses that
-// parse the time token keep working.  "Error" counts violations of the BFS
+// "- Error =", "- Time  = <ms>ms" (:206-208, :297-300) and the trailing
+// attribute reset (:311), so harnesses that parse the time token
+// (plot.py:32) keep working.  "Error" counts violations of the BFS
 // certificate (dfq_validate_bfs: level and parent-tree rules over every
 // vertex and edge) instead of re-running a CPU BFS.
 #include <cstdio>
@@ -65,7 +66,8 @@ int main(int argc, char **argv) {
     return die("dfq_graph_create", rc);
   }
   std::vector<uint32_t> level(csr->n), parent(csr->n);
-  if (!json) printf("%s\n", path.c_str());
+  // the graph's name in bold yellow, as the reference's first line (:183)
+  if (!json) printf("\033[0;1;33m%s\033[0;1m\n", path.c_str());
   bool first = true;
   for (std::string answer; answer != "y";) {
     // Source prompt; unlike the reference an invalid source in the file is an
@@ -98,7 +100,11 @@ int main(int argc, char **argv) {
     }
     dfq_validate_report rep;
     rc = dfq_validate_bfs(csr, source, level.data(), parent.data(), &rep);
-    if (rc != DFQ_OK) return die("dfq_validate_bfs", rc);
+    if (rc != DFQ_OK) {
+      dfq_graph_destroy(g);
+      dfq_csr_free(csr);
+      return die("dfq_validate_bfs", rc);
+    }
     uint64_t edges = 0;
     dfq_reached_work(csr, level.data(), &edges, nullptr, nullptr);
     const uint32_t dist = rep.depth ? rep.depth - 1 : 0;
@@ -111,7 +117,7 @@ int main(int argc, char **argv) {
       printf("- Dist  = %u\n", dist);
       printf("- Touch = %g %% (%llu / %u)\n", (double)rep.reached / (double)csr->n * 100.0,
              (unsigned long long)rep.reached, csr->n);
-      if (rep.errors) printf("- Error = \033[31m%llu\033[0m\n", (unsigned long long)rep.errors);
+      if (rep.errors) printf("- Error = \033[31m%llu\033[0;1m\n", (unsigned long long)rep.errors);
       else printf("- Error = 0\n");
       printf("- Time  = %gms\n", st.kernel_ms);
     }
@@ -120,6 +126,11 @@ int main(int argc, char **argv) {
     fflush(stdout);
     if (!(std::cin >> answer)) break;
   }
+  // The text report ends with an attribute reset after the last line, as the
+  // reference's does (:311): harnesses take the time from the next-to-last
+  // whitespace token (plot.py:32, `stdout.split()[-2][:-2]`).
+  if (!json) printf("\033[0m");
+  fflush(stdout);
   dfq_graph_destroy(g);
   dfq_csr_free(csr);
   return 0;

@@ -34,7 +34,12 @@ using namespace dfq;
 
 namespace {
 
+// Largest CSR entry count the library accepts anywhere (the device engine's
+// barrier sums are 40-bit fields); also keeps every byte count below 2^64.
+constexpr uint64_t kHostMaxEntries = (1ull << 40) - 1;
+
 dfq_csr *csr_alloc(uint32_t n, uint64_t m) {
+  if (m > kHostMaxEntries || n == 0xffffffffu) return nullptr;
   dfq_csr *g = (dfq_csr *)calloc(1, sizeof(dfq_csr));
   if (!g) return nullptr;
   g->n = n;
@@ -175,14 +180,25 @@ inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c ==
 
 // Parses an unsigned decimal at p (after optional blanks on the same line).
 // Returns false when no digit is found before the line end / buffer end.
+// Accumulates the digits at p into v; false when the number does not fit
+// in 64 bits (the digits are consumed either way).
+inline bool read_digits(const char *&p, const char *end, uint64_t &v) {
+  bool fits = true;
+  v = 0;
+  while (p < end && *p >= '0' && *p <= '9') {
+    const uint64_t d = (uint64_t)(*p - '0');
+    if (v > (UINT64_MAX - d) / 10) fits = false;
+    else v = v * 10 + d;
+    ++p;
+  }
+  return fits;
+}
+
 inline bool parse_u64(const char *&p, const char *end, uint64_t &val) {
   while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) ++p;
   if (p >= end || *p < '0' || *p > '9') return false;
   uint64_t v = 0;
-  while (p < end && *p >= '0' && *p <= '9') {
-    v = v * 10 + (uint64_t)(*p - '0');
-    ++p;
-  }
+  if (!read_digits(p, end, v)) return false;
   val = v;
   return true;
 }
@@ -197,10 +213,7 @@ inline bool next_u64(const char *&p, const char *end, uint64_t &val, bool &negat
   }
   if (p >= end || *p < '0' || *p > '9') return false;
   uint64_t v = 0;
-  while (p < end && *p >= '0' && *p <= '9') {
-    v = v * 10 + (uint64_t)(*p - '0');
-    ++p;
-  }
+  if (!read_digits(p, end, v)) return false;
   if (p < end && !is_space(*p)) return false;
   val = v;
   return true;
@@ -253,6 +266,7 @@ int dfq_csr_from_arrays(const int64_t *row_offsets, const uint32_t *col_idx, uin
                         uint64_t m, dfq_csr **out) {
   if (!out || !row_offsets || (m && !col_idx)) return DFQ_ERR_INVALID_ARG;
   *out = nullptr;
+  if (m > kHostMaxEntries || n == 0xffffffffu) return DFQ_ERR_RANGE;
   if (row_offsets[0] != 0 || (uint64_t)row_offsets[n] != m) return DFQ_ERR_RANGE;
   for (uint32_t v = 0; v < n; ++v)
     if (row_offsets[v + 1] < row_offsets[v]) return DFQ_ERR_RANGE;
@@ -268,9 +282,24 @@ int dfq_csr_from_arrays(const int64_t *row_offsets, const uint32_t *col_idx, uin
 
 // ---- SNAP edge list -----------------------------------------------------------
 
+namespace {
+int load_snap(const char *path, uint32_t flags, dfq_csr **out);
+}
+
 int dfq_load_snap(const char *path, uint32_t flags, dfq_csr **out) {
   if (!path || !out) return DFQ_ERR_INVALID_ARG;
   *out = nullptr;
+  try {
+    return load_snap(path, flags, out);
+  } catch (const std::bad_alloc &) {
+    return DFQ_ERR_NOMEM;
+  }
+}
+
+}  // extern "C"
+
+namespace {
+int load_snap(const char *path, uint32_t flags, dfq_csr **out) {
   MappedFile f;
   if (!f.open(path)) return DFQ_ERR_IO;
   const char *p = f.data, *end = f.data + f.size;
@@ -324,6 +353,9 @@ int dfq_load_snap(const char *path, uint32_t flags, dfq_csr **out) {
   }
   return dfq_csr_from_edges(src.data(), dst.data(), src.size(), n, flags, 0, out);
 }
+}  // namespace
+
+extern "C" {
 
 int dfq_save_snap(const char *path, const dfq_csr *g) {
   if (!path || !g) return DFQ_ERR_INVALID_ARG;
@@ -354,36 +386,48 @@ int dfq_load_dat(const char *path, dfq_csr **out, uint32_t *source_out) {
   if (!next_u64(p, end, n, neg) || neg) return DFQ_ERR_FORMAT;
   if (!next_u64(p, end, m, neg) || neg) return DFQ_ERR_FORMAT;
   if (!next_u64(p, end, s, neg)) return DFQ_ERR_FORMAT;
-  if (n > 0xFFFFFFFEull) return DFQ_ERR_RANGE;
+  if (n > 0xFFFFFFFEull || m > kHostMaxEntries) return DFQ_ERR_RANGE;
   if (neg || (n > 0 && s >= n) || (n == 0 && s != 0)) return DFQ_ERR_RANGE;
-  std::vector<uint64_t> start(n), len(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    if (!next_u64(p, end, start[i], neg) || neg) return DFQ_ERR_FORMAT;
-    if (!next_u64(p, end, len[i], neg) || neg) return DFQ_ERR_FORMAT;
-    if (start[i] + len[i] > m) return DFQ_ERR_RANGE;
+  // 2n + 2m more tokens follow, each at least one digit and one separator:
+  // a header that promises more than the file can hold is refused before
+  // anything is allocated for it
+  const uint64_t room = (uint64_t)(end - p) / 2 + 1;
+  if (n > room || m > room || n + m > room) return DFQ_ERR_FORMAT;
+  try {
+    std::vector<uint64_t> start(n), len(n);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      if (!next_u64(p, end, start[i], neg) || neg) return DFQ_ERR_FORMAT;
+      if (!next_u64(p, end, len[i], neg) || neg) return DFQ_ERR_FORMAT;
+      // row [start, start + len) inside edge[0, m), written so it cannot wrap
+      if (len[i] > m || start[i] > m - len[i]) return DFQ_ERR_RANGE;
+      // each step adds at most m < 2^40 to a total below 2^40: no wrap
+      total += len[i];
+      if (total > kHostMaxEntries) return DFQ_ERR_RANGE;
+    }
+    std::vector<uint32_t> edge(m);
+    for (uint64_t i = 0; i < m; ++i) {
+      uint64_t d, w;
+      if (!next_u64(p, end, d, neg) || neg) return DFQ_ERR_FORMAT;
+      if (!next_u64(p, end, w, neg)) return DFQ_ERR_FORMAT;
+      if (d >= n) return DFQ_ERR_RANGE;
+      edge[i] = (uint32_t)d;
+    }
+    // Rows may overlap or leave gaps in the reference layout; copy them out
+    // into a contiguous CSR in vertex order.
+    dfq_csr *g = csr_alloc((uint32_t)n, total);
+    if (!g) return DFQ_ERR_NOMEM;
+    g->row_offsets[0] = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      g->row_offsets[i + 1] = g->row_offsets[i] + (int64_t)len[i];
+      if (len[i]) memcpy(g->col_idx + g->row_offsets[i], edge.data() + start[i], sizeof(uint32_t) * len[i]);
+    }
+    *out = g;
+    if (source_out) *source_out = (uint32_t)s;
+    return DFQ_OK;
+  } catch (const std::bad_alloc &) {
+    return DFQ_ERR_NOMEM;
   }
-  std::vector<uint32_t> edge(m);
-  for (uint64_t i = 0; i < m; ++i) {
-    uint64_t d, w;
-    if (!next_u64(p, end, d, neg) || neg) return DFQ_ERR_FORMAT;
-    if (!next_u64(p, end, w, neg)) return DFQ_ERR_FORMAT;
-    if (d >= n) return DFQ_ERR_RANGE;
-    edge[i] = (uint32_t)d;
-  }
-  // Rows may overlap or leave gaps in the reference layout; copy them out
-  // into a contiguous CSR in vertex order.
-  uint64_t total = 0;
-  for (uint64_t i = 0; i < n; ++i) total += len[i];
-  dfq_csr *g = csr_alloc((uint32_t)n, total);
-  if (!g) return DFQ_ERR_NOMEM;
-  g->row_offsets[0] = 0;
-  for (uint64_t i = 0; i < n; ++i) {
-    g->row_offsets[i + 1] = g->row_offsets[i] + (int64_t)len[i];
-    if (len[i]) memcpy(g->col_idx + g->row_offsets[i], edge.data() + start[i], sizeof(uint32_t) * len[i]);
-  }
-  *out = g;
-  if (source_out) *source_out = (uint32_t)s;
-  return DFQ_OK;
 }
 
 int dfq_save_dat(const char *path, const dfq_csr *g, uint32_t source) {
@@ -429,6 +473,18 @@ int dfq_load_csr_bin(const char *path, dfq_csr **out) {
   uint64_t m = 0;
   if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, kCsrMagic, 8) != 0 ||
       fread(&n, sizeof(uint32_t), 1, fp) != 1 || fread(&m, sizeof(uint64_t), 1, fp) != 1) {
+    fclose(fp);
+    return DFQ_ERR_FORMAT;
+  }
+  // the header's sizes must match the file's before anything is allocated
+  struct stat st;
+  const uint64_t head = 8 + sizeof(uint32_t) + sizeof(uint64_t);
+  if (fstat(fileno(fp), &st) != 0) {
+    fclose(fp);
+    return DFQ_ERR_IO;
+  }
+  if (n == 0xffffffffu || m > kHostMaxEntries ||
+      (uint64_t)st.st_size != head + sizeof(int64_t) * ((uint64_t)n + 1) + sizeof(uint32_t) * m) {
     fclose(fp);
     return DFQ_ERR_FORMAT;
   }

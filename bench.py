@@ -60,12 +60,14 @@ MODES = {
 
 def run_mode(dfq, g, srcs, work, steps, warmup, d_level, launch="seq"):
     """Per-source mean ms over the timed steps, and wall ms per step.
-    launch "seq": one launch per step, each source's device-clock time
-    scaled so the step's shares add up to the launch's event time; "each":
-    one launch per source, each bracketed by its own events."""
+    launch "seq": one launch per step, search i writing its levels at
+    d_level + i * n (the output the bench checks afterwards), each source's
+    device-clock time scaled so the step's shares add up to the launch's
+    event time; "each": one launch per source, each bracketed by its own
+    events."""
     def step():
         if launch == "seq":
-            st, launch_ms = g.bfs_device_seq(srcs, d_level.ptr)
+            st, launch_ms = g.bfs_device_seq(srcs, d_level.ptr, level_stride=g.n)
             dev = np.array([x.kernel_ms for x in st])
             return dev * (launch_ms / dev.sum())
         return np.array([x.kernel_ms for x in g.bfs_device_many(srcs, d_level.ptr)])
@@ -78,16 +80,22 @@ def run_mode(dfq, g, srcs, work, steps, warmup, d_level, launch="seq"):
         times[k] = step()
     dfq.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
-    t = times.mean(axis=0)
+    return dict(t=times.mean(axis=0), wall_ms_per_step=wall_ms)
+
+
+def price(r, work, n):
+    """GTEPS and roofline of per-source times r["t"] (ms) given each
+    search's (E_r, V_r)."""
+    t = r["t"]
     e_r = np.array([w[0] for w in work], dtype=np.float64)
     v_r = np.array([w[1] for w in work], dtype=np.float64)
     gteps = (e_r / 2) / (t * 1e-3) / 1e9
     hmean = len(gteps) / np.sum(1.0 / gteps)
-    b_alg = 4 * e_r + 16 * v_r + 4 * g.n
+    b_alg = 4 * e_r + 16 * v_r + 4 * n
     achieved = b_alg.sum() / (t.sum() * 1e-3) / 1e9
     return dict(gteps_hmean=float(hmean), gteps_median=float(np.median(gteps)),
                 ms_per_bfs_mean=float(t.mean()), ms_per_bfs_median=float(np.median(t)),
-                wall_ms_per_step=wall_ms, achieved_gbs=float(achieved),
+                wall_ms_per_step=r["wall_ms_per_step"], achieved_gbs=float(achieved),
                 b_alg_mean=float(b_alg.mean()), b_alg_step=float(b_alg.sum()),
                 kernel_ms_total=float(t.sum()))
 
@@ -169,7 +177,7 @@ def batch_scales(count, seed, lo=10, hi=20, exponent=1.5):
     return rng.choice(s, size=count, p=w / w.sum())
 
 
-def run_batch(args, dfq, info):
+def run_batch(args, dfq):
     """Batched form: many graphs x sources in one persistent launch against
     the same queries issued as back-to-back plain launches."""
     t0 = time.perf_counter()
@@ -184,7 +192,12 @@ def run_batch(args, dfq, info):
     nq = len(gi)
     log(f"{len(graphs)} graphs, {sum(g.n for g in graphs)} vertices, {sum(g.m for g in graphs)} "
         f"entries, {nq} queries, built in {time.perf_counter() - t0:.1f}s")
-    b = dfq.Batch(graphs)
+    with dfq.Batch(graphs) as b:
+        return batch_on(args, dfq, b, graphs, gi, so)
+
+
+def batch_on(args, dfq, b, graphs, gi, so):
+    nq = len(gi)
     b.set_queries(gi, so)
     t0 = time.perf_counter()
     tune = b.autotune(repeats=1)
@@ -194,11 +207,12 @@ def run_batch(args, dfq, info):
         log("  bucket", e["bucket"], "queries", e["queries"], e["plan"], f"{e['sub_batch_ms']:.3f} ms")
     for _ in range(args.warmup):
         b.run()
-    ts = []
+    ts, replays = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = b.run()
         ts.append(st.kernel_ms)
+        replays.append(bool(st.graph_replay))
     wall_batch = (time.perf_counter() - t0) * 1e3 / args.steps
     # back-to-back plain launches of the same queries (one resident graph each)
     dgs = [dfq.DeviceGraph(g) for g in graphs]
@@ -221,6 +235,9 @@ def run_batch(args, dfq, info):
         ref = dgs[gi[q]].bfs(int(so[q])).level
         exact &= bool(np.array_equal(lv, ref))
         edges += graphs[gi[q]].reached_work(lv)[0] / 2
+    for dg in dgs:
+        dg.close()
+    dbuf.free()
     kernel_ms = float(np.mean(ts))
     qps = nq / (kernel_ms * 1e-3)
     plain_qps = nq / (plain_wall * 1e-3)
@@ -232,8 +249,12 @@ def run_batch(args, dfq, info):
         "config": {"workload": f"{len(graphs)} undirected RMAT graphs, edgefactor 8, scales "
                                f"10..20 with p(n)~n^-1.5, {args.batch_sources} sources each",
                    "queries": nq, "vertices_total": int(sum(g.n for g in graphs)),
-                   "graph_replay": bool(st.graph_replay), "block_size": st.block_size},
+                   "graph_replay": all(replays), "block_size": st.block_size},
         "batch_kernel_ms": round(kernel_ms, 3),
+        # every query of a batch completes with its launch: a query's
+        # latency is the launch time, the same for all of them
+        "latency_p50_ms": round(float(np.percentile(ts, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(ts, 99)), 3),
         "plain_back_to_back_ms": round(plain_wall, 3),
         "plain_kernel_sum_ms": round(plain_kernel, 3),
         "plain_queries_per_s": round(plain_qps, 1),
@@ -245,7 +266,7 @@ def run_batch(args, dfq, info):
                       "medium_max": e["plan"]["medium_max"],
                       "direction_opt": e["plan"]["direction_opt"]} for e in tune["buckets"]],
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 SERVE_SCALES = (18, 19, 20, 21, 22, 22, 23, 24)
@@ -266,7 +287,7 @@ def serve_trace(graphs, scales, nq, rate_qps, seed):
     return gi, so, arrival
 
 
-def run_serve(args, dfq, info):
+def run_serve(args, dfq):
     """Served form: resident graphs, one persistent launch of XCD teams,
     continuous batching under Poisson arrivals, against the same queries
     issued as back-to-back plain launches."""
@@ -300,7 +321,11 @@ def run_serve(args, dfq, info):
     dbuf.free()
     del dgs
 
-    srv = dfq.Server(graphs, xcds_per_team=1)
+    with dfq.Server(graphs, xcds_per_team=1) as srv:
+        return serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full)
+
+
+def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full):
     # team width: the closed-loop rate of the plain sample at each shape
     sweep = {}
     exact = True
@@ -371,7 +396,7 @@ def run_serve(args, dfq, info):
         "shape_sweep_qps": {str(k): round(v, 1) for k, v in sweep.items()},
         "results_exact_vs_plain": exact,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -388,6 +413,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the Amazon-like and grid lines")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched and served objects of the headline line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy of BYTES (counter calibration)")
     ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain",
@@ -421,49 +448,60 @@ def main():
                           "GBps_read_plus_write": round(2 * args.copy_calibration / min(ms) / 1e6, 1)}))
         return
     if args.workload == "batch":
-        run_batch(args, dfq, info)
+        print(json.dumps(run_batch(args, dfq)), flush=True)
         return
     if args.workload == "serve":
-        run_serve(args, dfq, info)
+        print(json.dumps(run_serve(args, dfq)), flush=True)
         return
     t0 = time.perf_counter()
     csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)
     log(f"graph n={csr.n} m={csr.m} built in {time.perf_counter() - t0:.1f}s")
     srcs = csr.pick_sources(args.sources, seed=2022)
-    g = dfq.DeviceGraph(csr)
-    d_level = dfq.DeviceBuffer(4 * csr.n)
-
-    # one checked run per source: the certificate (levels + parent tree) and
-    # the reached work that prices GTEPS and B_alg
-    g.set_config(**MODES[args.mode])
-    work, levels = [], []
-    for s in srcs:
-        r = g.bfs(int(s), parents=True)
-        rep = csr.validate(int(s), r.level, r.parent)
-        if rep["errors"]:
-            raise SystemExit(f"BFS certificate failed for source {s}: {rep}")
-        work.append(csr.reached_work(r.level))
-        levels.append(r.level)
-    log("certificate ok on all sources")
-
-    res = run_mode(dfq, g, srcs, work, args.steps, args.warmup, d_level)
-    log(args.mode, res)
-    each = run_mode(dfq, g, srcs, work, args.steps, args.warmup, d_level, launch="each")
-    log(args.mode, "one launch per search", each)
-    extra = {"one_launch_per_search": dict(gteps_hmean=round(each["gteps_hmean"], 3),
-                                           ms_per_bfs_mean=round(each["ms_per_bfs_mean"], 4),
-                                           roofline_frac=round(each["achieved_gbs"] / HBM_PEAK_GBS, 4))}
-    if args.also != "none" and args.also != args.mode:
-        g.set_config(**MODES[args.also])
-        other = run_mode(dfq, g, srcs, work, args.steps, args.warmup, d_level)
-        log(args.also, other)
-        extra[args.also] = dict(gteps_hmean=round(other["gteps_hmean"], 3),
-                                ms_per_bfs_mean=round(other["ms_per_bfs_mean"], 4),
-                                roofline_frac=round(other["achieved_gbs"] / HBM_PEAK_GBS, 4))
+    ns = len(srcs)
+    out = {}
+    with dfq.DeviceGraph(csr) as g, dfq.DeviceBuffer(4 * csr.n * ns) as d_level:
         g.set_config(**MODES[args.mode])
+        # Untimed: the same sequence launch with parents, every search's
+        # parent tree certified (levels + tree rules over every vertex/edge).
+        with dfq.DeviceBuffer(4 * csr.n * ns) as d_parent:
+            g.bfs_device_seq(srcs, d_level.ptr, level_stride=csr.n, d_parent=d_parent.ptr,
+                             parent_stride=csr.n)
+            for i, s in enumerate(srcs):
+                lv = d_level.to_numpy(count=csr.n, offset=i * csr.n)
+                pa = d_parent.to_numpy(count=csr.n, offset=i * csr.n)
+                rep = csr.validate(int(s), lv, pa)
+                if rep["errors"]:
+                    raise SystemExit(f"BFS certificate failed for source {s}: {rep}")
+        log("certificate (levels + parent tree) ok on all sources")
+
+        res = run_mode(dfq, g, srcs, None, args.steps, args.warmup, d_level)
+        # the levels of the last timed launch: certificate and reached work
+        # of every search, and (cpu_baseline) bit-exactness on a sample
+        levels = [d_level.to_numpy(count=csr.n, offset=i * csr.n) for i in range(ns)]
+        work = []
+        for s, lv in zip(srcs, levels):
+            rep = csr.validate(int(s), lv)
+            if rep["errors"]:
+                raise SystemExit(f"timed launch: level certificate failed for source {s}: {rep}")
+            work.append(csr.reached_work(lv))
+        res = price(res, work, csr.n)
+        log(args.mode, res)
+        each = price(run_mode(dfq, g, srcs, None, args.steps, args.warmup, d_level, launch="each"),
+                     work, csr.n)
+        log(args.mode, "one launch per search", each)
+        extra = {"one_launch_per_search": dict(gteps_hmean=round(each["gteps_hmean"], 3),
+                                               ms_per_bfs_mean=round(each["ms_per_bfs_mean"], 4),
+                                               roofline_frac=round(each["achieved_gbs"] / HBM_PEAK_GBS, 4))}
+        if args.also != "none" and args.also != args.mode:
+            g.set_config(**MODES[args.also])
+            other = price(run_mode(dfq, g, srcs, None, args.steps, args.warmup, d_level), work, csr.n)
+            log(args.also, other)
+            extra[args.also] = dict(gteps_hmean=round(other["gteps_hmean"], 3),
+                                    ms_per_bfs_mean=round(other["ms_per_bfs_mean"], 4),
+                                    roofline_frac=round(other["achieved_gbs"] / HBM_PEAK_GBS, 4))
 
     traffic = load_traffic(f"traffic_rmat{args.scale}_{args.mode}.json")
-    out = {
+    out.update({
         "metric": "GTEPS + % HBM roofline (RMAT-22)",
         "value": round(res["gteps_hmean"], 3),
         "unit": "GTEPS",
@@ -490,12 +528,28 @@ def main():
         "b_alg_bytes_mean": int(res["b_alg_mean"]),
         "b_alg_bytes_per_launch": int(res["b_alg_step"]),
         "launch_ms_mean": round(res["kernel_ms_total"], 4),
-    }
+        "timed_output_checked": "levels of the last timed launch (dfq_bfs_seq_kernel, stride n): "
+                                "certificate on every search; parents of an untimed launch of the "
+                                "same kernel and config certified on every search",
+    })
     out.update(extra)
     if not args.no_extra:
         out["other_configs"] = other_configs(dfq)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(csr, srcs, levels, args.cpu_budget)
+    del levels
+    if not args.no_forms:
+        b = run_batch(args, dfq)
+        out["batch"] = {k: b[k] for k in ("value", "unit", "plain_queries_per_s", "speedup_vs_plain",
+                                          "levels_bit_exact_vs_plain", "latency_p50_ms",
+                                          "latency_p99_ms", "gteps_batch")}
+        out["batch"]["config"] = b["config"]
+        sv = run_serve(args, dfq)
+        out["serve"] = {k: sv[k] for k in ("value", "unit", "plain_queries_per_s", "speedup_vs_plain",
+                                           "results_exact_vs_plain", "gteps_served")}
+        out["serve"]["poisson"] = {k: sv["poisson"][k] for k in ("rate_qps", "queries", "achieved_qps",
+                                                                 "p50_ms", "p99_ms")}
+        out["serve"]["config"] = sv["config"]
     print(json.dumps(out), flush=True)
     if args.then_copy:
         a, b = dfq.DeviceBuffer(args.then_copy), dfq.DeviceBuffer(args.then_copy)

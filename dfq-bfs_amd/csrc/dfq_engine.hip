@@ -321,7 +321,9 @@ int dfq_get_device_info(dfq_device_info *info) {
   info->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   info->hbm_bytes = prop.totalGlobalMem;
   info->l2_bytes = (uint64_t)prop.l2CacheSize;
-  info->xcc_count = 0;
+  int xcc = 0;
+  if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess) xcc = 0;
+  info->xcc_count = xcc;
   strncpy(info->arch, prop.gcnArchName, sizeof(info->arch) - 1);
   return DFQ_OK;
 }

@@ -15,8 +15,10 @@ optionally, a BFS-tree parent array.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -56,6 +58,28 @@ if not os.path.exists(LIB_PATH):
 lib = C.CDLL(LIB_PATH)
 
 LAUNCH_PLAIN, LAUNCH_COOPERATIVE, LAUNCH_PLAIN_ABANDON = 0, 1, 2
+
+
+# Every device object still open when the interpreter exits is closed by an
+# exit handler, newest first, while the library and its HIP runtime are
+# whole: left to __del__ during interpreter teardown, the frees could run
+# after a profiler attached to the process has shut down.
+_open_objects: "list[weakref.ref]" = []
+
+
+def _track(obj):
+    _open_objects.append(weakref.ref(obj))
+    if len(_open_objects) > 256:
+        _open_objects[:] = [r for r in _open_objects if r() is not None]
+
+
+@atexit.register
+def close_all():
+    """Closes every open DeviceGraph, DeviceBuffer, Batch and Server."""
+    while _open_objects:
+        obj = _open_objects.pop()()
+        if obj is not None:
+            obj.close()
 
 
 class DfqError(RuntimeError):
@@ -330,6 +354,7 @@ class DeviceGraph:
         h = _P()
         check(lib.dfq_graph_create(ro, ci, n, m, C.byref(h)), "dfq_graph_create")
         self._h = h
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -425,13 +450,19 @@ class DeviceBuffer:
         check(lib.dfq_device_alloc(nbytes, C.byref(p)), "dfq_device_alloc")
         self.ptr = p.value
         self.nbytes = nbytes
+        _track(self)
 
-    def to_numpy(self, dtype=np.uint32, count=None) -> np.ndarray:
+    def to_numpy(self, dtype=np.uint32, count=None, offset: int = 0) -> np.ndarray:
+        """`count` elements from element `offset` on (all the rest by default)."""
         dt = np.dtype(dtype)
-        count = self.nbytes // dt.itemsize if count is None else count
+        avail = self.nbytes // dt.itemsize - offset
+        count = avail if count is None else count
+        if offset < 0 or count < 0 or count > avail:
+            raise ValueError("range outside the buffer")
         out = np.empty(count, dt)
-        check(lib.dfq_copy_to_host(out.ctypes.data_as(_P), C.c_void_p(self.ptr), out.nbytes),
-              "dfq_copy_to_host")
+        if count:
+            check(lib.dfq_copy_to_host(out.ctypes.data_as(_P), C.c_void_p(self.ptr + offset * dt.itemsize),
+                                       out.nbytes), "dfq_copy_to_host")
         return out
 
     def from_numpy(self, a: np.ndarray):
@@ -446,7 +477,15 @@ class DeviceBuffer:
             lib.dfq_device_free(C.c_void_p(self.ptr))
             self.ptr = None
 
+    close = free
+
     def __del__(self):
+        self.free()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
         self.free()
 
 
@@ -522,6 +561,7 @@ class Batch:
         check(lib.dfq_batch_create(len(self.graphs), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
                                    _i64(ro), _u32(ci), C.byref(h)), "dfq_batch_create")
         self._h = h
+        _track(self)
         self.n_queries = 0
 
     def close(self):
@@ -672,6 +712,7 @@ class Server:
         check(lib.dfq_server_create(len(self.graphs), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
                                     _i64(ro), _u32(ci), C.byref(cfg), C.byref(h)), "dfq_server_create")
         self._h = h
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):

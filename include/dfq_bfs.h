@@ -10,10 +10,12 @@
  *   :273-277     cudaMemPrefetchAsync                     device copies, no managed
  *                                                         memory)
  *   :98-179      __global__ CUDA_BFS_KERNEL_CG_NEW(Node*, dfq_bfs / dfq_bfs_device:
- *                int*, short*, bool*, int*)              one persistent cooperative
- *   :255-271     occupancy query, grid = SMs x blocks     launch per BFS on gfx950,
- *   :279-288     cudaEvent timing around                  hipEvent timing in
- *                cudaLaunchCooperativeKernel + sync       dfq_stats.kernel_ms
+ *                int*, short*, bool*, int*)              one persistent launch per
+ *   :255-271     occupancy query, grid = SMs x blocks     BFS on gfx950 (plain by
+ *   :279-288     cudaEvent timing around                  default: the grid assembles
+ *                cudaLaunchCooperativeKernel + sync       itself, cooperative fallback;
+ *                                                         see launch_mode), hipEvent
+ *                                                         timing in dfq_stats.kernel_ms
  *   :312-316     cudaFree                                dfq_graph_destroy
  *
  * Output convention: level[v] is the hop distance from the source (uint32,

@@ -14,6 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "dfq-bfs_amd")
 if PKG not in sys.path:
     sys.path.insert(0, PKG)
+if ROOT not in sys.path:
+    sys.path.insert(1, ROOT)  # bench.py: tests reuse its workload definitions
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 UNREACHED = 0xFFFFFFFF

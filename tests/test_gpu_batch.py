@@ -109,3 +109,50 @@ def test_batch_requery_and_errors(dfq, batch_setup):
         for q, (g, s) in enumerate(((2, 5), (0, 1))):
             with dfq.DeviceGraph(gs[g]) as dg:
                 np.testing.assert_array_equal(b.levels(q), dg.bfs(s).level)
+
+
+def test_batch_config4_full_workload(dfq, oracle):
+    """Config 4 at its stated size, built exactly as bench.py builds it: 512
+    undirected RMAT graphs (edgefactor 8, scales 10..20, p(n) ~ n^-1.5), 4
+    sources each, 2,048 queries in one launch.  After autotuning, the first
+    run captures the hipGraph and every later run replays it; every query's
+    levels equal the plain engine's bit for bit, and a sample equals the
+    oracle's."""
+    import bench
+    scales = bench.batch_scales(512, seed=4)
+    gs = [dfq.Csr.rmat(int(s), 8, 0.57, 0.19, 0.19, seed=1000 + i) for i, s in enumerate(scales)]
+    gi, so = [], []
+    for i, g in enumerate(gs):
+        for s in g.pick_sources(4, seed=i):
+            gi.append(i)
+            so.append(int(s))
+    gi, so = np.array(gi), np.array(so)
+    assert len(gi) == 2048 and min(scales) >= 10 and max(scales) <= 20
+    say("config 4:", len(gs), "graphs,", sum(g.n for g in gs), "vertices,", len(gi), "queries")
+    with dfq.Batch(gs) as b:
+        b.set_queries(gi, so)
+        tune = b.autotune(repeats=1)
+        say("autotune: block", tune["block_size"], "buckets", len(tune["buckets"]))
+        runs = [b.run() for _ in range(3)]
+        assert all(st.queries == 2048 for st in runs)
+        assert [bool(st.graph_replay) for st in runs[1:]] == [True, True], \
+            [st.graph_replay for st in runs]
+        by_graph = {}
+        for q in range(len(gi)):
+            by_graph.setdefault(int(gi[q]), []).append(q)
+        checked = 0
+        for g, qs in by_graph.items():
+            with dfq.DeviceGraph(gs[g]) as dg:
+                dg.set_config(direction_opt=1)
+                for q in qs:
+                    ref = dg.bfs(int(so[q]))
+                    lv = b.levels(q)
+                    np.testing.assert_array_equal(lv, ref.level, err_msg=f"query {q} graph {g}")
+                    depth, reached = b.result(q)
+                    assert (depth, reached) == (ref.depth, ref.reached)
+                    if q % 32 == 0:
+                        np.testing.assert_array_equal(
+                            lv, oracle.bfs(gs[g].row_offsets, gs[g].col_idx, int(so[q])),
+                            err_msg=f"oracle: query {q} graph {g}")
+                        checked += 1
+        say("all 2048 queries equal the plain engine;", checked, "equal the oracle")

@@ -22,6 +22,14 @@ def check_result(dfq, csr, source, res, expect):
     assert res.depth == int(expect[expect != UNREACHED].max()) + 1
 
 
+def test_device_info(dfq):
+    info = dfq.device_info()
+    say("device", info)
+    assert info["cu_count"] > 0 and info["cooperative"] == 1
+    if info["arch"].startswith("gfx950"):
+        assert info["cu_count"] == 256 and info["xcc_count"] == 8
+
+
 @pytest.mark.parametrize("name", golden_names())
 def test_golden(dfq, oracle, name):
     gd = load_golden(name)
@@ -231,6 +239,41 @@ def test_rmat22_full_size(dfq, oracle):
             e, v, _ = csr.reached_work(r.level)
             e_ref = e_ref or e
             assert e == e_ref  # every source lies in the one giant component
+
+
+def test_rmat22_timed_kernel_bench_config(dfq, oracle):
+    """Exactly what bench.py times: the sequence kernel (one launch, 64
+    searches, search i writing at level + i * n) in the hybrid config on the
+    config-2 graph (RMAT-22, graph seed 22, sources seed 2022).  Eight
+    searches bit-exact against the oracle, the other 56 certified; the
+    parents of a second launch certified on all 64."""
+    import bench
+    csr = dfq.Csr.rmat(22, 16, seed=22)
+    srcs = csr.pick_sources(64, seed=2022)
+    n = csr.n
+    with dfq.DeviceGraph(csr) as g, dfq.DeviceBuffer(4 * n * 64) as d_level, \
+            dfq.DeviceBuffer(4 * n * 64) as d_parent:
+        g.set_config(**bench.MODES["hybrid"])
+        assert g.config().block_size == 1024 and g.config().direction_opt == 1
+        say("rmat22 sequence launch, 64 sources, hybrid, stride n")
+        stats, ms = g.bfs_device_seq(srcs, d_level.ptr, level_stride=n)
+        assert ms > 0 and sum(st.bottom_up_levels for st in stats) > 0
+        for i, s in enumerate(srcs):
+            lv = d_level.to_numpy(count=n, offset=i * n)
+            if i % 8 == 0:
+                say("  oracle compare, search", i, "source", int(s))
+                e = oracle.bfs(csr.row_offsets, csr.col_idx, int(s))
+                np.testing.assert_array_equal(lv, e, err_msg=f"search {i} source {s}")
+            else:
+                assert csr.validate(int(s), lv)["errors"] == 0, (i, int(s))
+            assert stats[i].reached == int((lv != UNREACHED).sum())
+            assert stats[i].depth == int(lv[lv != UNREACHED].max()) + 1
+        say("rmat22 sequence launch with parents")
+        g.bfs_device_seq(srcs, d_level.ptr, level_stride=n, d_parent=d_parent.ptr, parent_stride=n)
+        for i, s in enumerate(srcs):
+            lv = d_level.to_numpy(count=n, offset=i * n)
+            pa = d_parent.to_numpy(count=n, offset=i * n)
+            assert csr.validate(int(s), lv, pa)["errors"] == 0, (i, int(s))
 
 
 def test_grid_config3(dfq, oracle):

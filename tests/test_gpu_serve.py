@@ -180,3 +180,55 @@ def test_serve_restart_and_results_after_stop(dfq, serve_setup):
                 np.testing.assert_array_equal(srv.levels(t, int(gi[q])), expect[(gi[q], so[q])])
                 srv.release(t)
             assert srv.info()["in_flight"] == 0
+
+
+def test_serve_config5_resident_graphs(dfq, oracle):
+    """Config 5 graphs at their stated sizes, built exactly as bench.py
+    builds them: 8 resident undirected RMAT graphs of scales 18, 19, 20, 21,
+    22, 22, 23, 24 (edgefactor 16), queries picking graph i with p ~ 1/scale_i
+    and a uniform degree>0 source.  3,000 Poisson arrivals at about half the
+    served rate (many queries in flight): every query's reached count and
+    depth equal the plain engine's; then 48 queries served in a dense mix
+    give full level arrays equal to the plain engine's, and one per graph
+    equal to the oracle's."""
+    import bench
+    scales = bench.SERVE_SCALES
+    gs = [dfq.Csr.rmat(s, 16, 0.57, 0.19, 0.19, seed=5000 + i) for i, s in enumerate(scales)]
+    say("config 5:", sum(g.n for g in gs), "vertices,", sum(g.m for g in gs), "entries")
+    gi, so, arr = bench.serve_trace(gs, scales, 3000, 4000.0, seed=13)
+    g2, s2, _ = bench.serve_trace(gs, scales, 48, None, seed=17)
+    arr2 = np.arange(48) * 5.0  # 5 us apart: the whole sample in flight together
+    # the plain engine, one launch per query
+    plain = np.zeros((len(gi), 2), np.uint64)
+    plain_lv = [None] * len(g2)
+    with dfq.DeviceBuffer(4 * max(g.n for g in gs)) as buf:
+        for g in range(len(gs)):
+            with dfq.DeviceGraph(gs[g]) as dg:
+                dg.set_config(direction_opt=1)
+                for q in np.flatnonzero(gi == g):
+                    st = dg.bfs_device(int(so[q]), buf.ptr)
+                    plain[q] = (st.reached, st.depth)
+                for q in np.flatnonzero(g2 == g):
+                    plain_lv[q] = dg.bfs(int(s2[q])).level
+    assert all(lv is not None for lv in plain_lv)
+    with dfq.Server(gs, xcds_per_team=1) as srv:
+        srv.start()
+        say("server", srv.info())
+        recs, _ = srv.replay(gi, so, arr)
+        say("3000 Poisson queries: max in flight", max_overlap(recs))
+        assert max_overlap(recs) > 1
+        np.testing.assert_array_equal(recs["reached"], plain[:, 0])
+        np.testing.assert_array_equal(recs["depth"].astype(np.uint64), plain[:, 1])
+        recs2, outs = srv.replay(g2, s2, arr2, keep_levels=True)
+        say("48-query mix: max in flight", max_overlap(recs2))
+        srv.stop()
+    seen = set()
+    for q in range(len(g2)):
+        np.testing.assert_array_equal(outs[q], plain_lv[q], err_msg=f"query {q} graph {g2[q]}")
+        if int(g2[q]) not in seen:
+            seen.add(int(g2[q]))
+            g = gs[int(g2[q])]
+            say("oracle, graph", int(g2[q]), "source", int(s2[q]))
+            np.testing.assert_array_equal(outs[q], oracle.bfs(g.row_offsets, g.col_idx, int(s2[q])),
+                                          err_msg=f"oracle: query {q} graph {g2[q]}")
+    say("graphs checked against the oracle:", sorted(seen))

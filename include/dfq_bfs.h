@@ -143,6 +143,28 @@ int dfq_graph_is_symmetric(const dfq_graph *g);
 int dfq_bfs_device(dfq_graph *g, uint32_t source, uint32_t *d_level,
                    uint32_t *d_parent, dfq_stats *stats);
 
+/* ---- BFS for callers that cannot allocate ---------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops that must not call hipMalloc, harnesses that forbid
+ * it): the same BFS as dfq_bfs_device under the default configuration
+ * (top-down levels through the degree bins) over a CSR the caller already
+ * holds in device memory, d_row_offsets[n + 1] (int64) and d_col_idx[m].
+ * The library allocates nothing on this path: every scratch buffer of the
+ * search comes from one static device array of DFQ_NOALLOC_SCRATCH_BYTES
+ * (~20 MiB) per GPU, held until the process ends.  Limits: n <=
+ * DFQ_NOALLOC_MAX_VERTICES, m <= DFQ_NOALLOC_MAX_ENTRIES, else
+ * DFQ_ERR_CAPACITY.  The CSR is checked on the device first (offsets from 0
+ * to m, never decreasing, columns < n; DFQ_ERR_RANGE otherwise).  Outputs:
+ * d_level[n] required, d_parent[n] optional, as for dfq_bfs_device; runs on
+ * the current device's null stream and blocks until done.  Calls share the
+ * device's array: calls that overlap on one GPU race and must not be made. */
+#define DFQ_NOALLOC_MAX_VERTICES (1u << 18)
+#define DFQ_NOALLOC_MAX_ENTRIES (1ull << 23)
+#define DFQ_NOALLOC_SCRATCH_BYTES 21115904ull
+int dfq_bfs_device_noalloc(const int64_t *d_row_offsets, const uint32_t *d_col_idx,
+                           uint32_t n, uint64_t m, uint32_t source,
+                           uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats);
+
 /* count searches back to back on the graph's stream, outputs in device
  * memory (each search overwrites the previous one's), one wait at the end.
  * stats[i] belongs to sources[i]; its kernel_ms is bracketed by hipEvents

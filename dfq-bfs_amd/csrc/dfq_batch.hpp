@@ -413,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     a.bottom_up = (q.direction_opt && gr.symmetric) ? 1u : 0u;
     a.alpha = q.alpha;
     a.bu_exit = q.beta ? gr.n / q.beta : 0;
-    grid_bfs<BLOCK>(a, sh.grid, team);
+    grid_bfs<BLOCK>(a, SearchIo{a.level, a.parent, a.result, a.source}, sh.grid, team);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       b.results[qi].depth = b.grid.ctl->depth;
       b.results[qi].reached = b.grid.ctl->reached;

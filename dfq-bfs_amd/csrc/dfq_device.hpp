@@ -40,6 +40,15 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// A value every lane of the wave holds alike (read from one LDS word, say),
+// moved to a scalar register: branches on it are then scalar branches and
+// it takes no vector register.
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uniform(uint64_t x) {
+  const uint32_t lo = uniform((uint32_t)x), hi = uniform((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t bcast_lane(uint32_t x, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }
@@ -410,8 +419,8 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
     s_word[1] = ws & kPayloadMask;
   }
   __syncthreads();
-  if (with_side) *side_total = s_word[1];
-  return s_word[0];
+  if (with_side) *side_total = uniform(s_word[1]);
+  return uniform(s_word[0]);
 }
 
 // ---------------------------------------------------------------------------

@@ -81,6 +81,7 @@ struct dfq_graph {
   dfq_config cfg{};
   int grid = 0;
   int grid_seq = 0;  // the sequence kernel's grid (its own occupancy)
+  int grid_prof = 0; // the profiling kernel's grid (its own occupancy)
   int block = 0;
 };
 
@@ -98,11 +99,13 @@ int validate_config(const dfq_config &c) {
 
 static_assert(DFQ_PROFILE_FIELDS == kProfFields, "header and kernel disagree on the profile row");
 
-const void *kernel_for(uint32_t block) {
+// prof: the instantiation with the per-level counters compiled in (used by
+// dfq_bfs_profile only; the search kernels carry none of that code).
+const void *kernel_for(uint32_t block, bool prof = false) {
   switch (block) {
-    case 1024: return (const void *)&dfq_bfs_kernel<1024>;
-    case 512: return (const void *)&dfq_bfs_kernel<512>;
-    default: return (const void *)&dfq_bfs_kernel<256>;
+    case 1024: return prof ? (const void *)&dfq_bfs_kernel<1024, true> : (const void *)&dfq_bfs_kernel<1024>;
+    case 512: return prof ? (const void *)&dfq_bfs_kernel<512, true> : (const void *)&dfq_bfs_kernel<512>;
+    default: return prof ? (const void *)&dfq_bfs_kernel<256, true> : (const void *)&dfq_bfs_kernel<256>;
   }
 }
 
@@ -130,9 +133,15 @@ int configure(dfq_graph *g, const dfq_config &cfg) {
                                                         (int)cfg.block_size, 0));
   if (per_cu_seq <= 0) return DFQ_ERR_COOPERATIVE;
   per_cu_seq = std::min(per_cu_seq, per_cu);
+  int per_cu_prof = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_prof, kernel_for(cfg.block_size, true),
+                                                        (int)cfg.block_size, 0));
+  if (per_cu_prof <= 0) return DFQ_ERR_COOPERATIVE;
+  per_cu_prof = std::min(per_cu_prof, per_cu);
   g->cfg = cfg;
   g->grid = prop.multiProcessorCount * per_cu;
   g->grid_seq = prop.multiProcessorCount * per_cu_seq;
+  g->grid_prof = prop.multiProcessorCount * per_cu_prof;
   g->block = (int)cfg.block_size;
   return DFQ_OK;
 }
@@ -242,10 +251,11 @@ void launch_kind(dfq_graph *g, BfsArgs &a, bool plain) {
 // assembling its grid itself (and giving it up when it cannot).
 hipError_t launch_search(dfq_graph *g, BfsArgs &a) {
   void *args[] = {&a};
-  if (a.plain)
-    return hipLaunchKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0, g->stream);
-  return hipLaunchCooperativeKernel(kernel_for(g->cfg.block_size), dim3(g->grid), dim3(g->block), args, 0,
-                                    g->stream);
+  const bool prof = a.prof != nullptr;
+  const void *k = kernel_for(g->cfg.block_size, prof);
+  const int grid = prof ? g->grid_prof : g->grid;
+  if (a.plain) return hipLaunchKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
+  return hipLaunchCooperativeKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
 }
 
 int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent, dfq_stats *stats,
@@ -277,7 +287,7 @@ int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent
     stats->kernel_ms = ms;
     stats->depth = c.depth;
     stats->reached = c.reached;
-    stats->grid_blocks = (uint32_t)g->grid;
+    stats->grid_blocks = (uint32_t)(prof ? g->grid_prof : g->grid);
     stats->block_size = (uint32_t)g->block;
     stats->bottom_up_levels = c.bu_levels;
   }

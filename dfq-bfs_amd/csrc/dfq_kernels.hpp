@@ -125,6 +125,17 @@ struct BfsArgs {
   uint64_t m;            // CSR entries
 };
 
+// What changes from one search to the next on the same resident graph:
+// kept apart from BfsArgs, so a launch that runs many searches leaves its
+// argument block untouched (constant, re-read from the kernel arguments
+// instead of held in registers).
+struct SearchIo {
+  uint32_t *level;   // [n]
+  uint32_t *parent;  // [n] or null
+  uint64_t *result;  // [kResultWords] or null
+  uint32_t source;
+};
+
 // Independent 64-edge steps a wave keeps in flight in the expansions.
 constexpr int kUnroll = 4;
 constexpr uint32_t kItem = kWave * kUnroll;  // CSR entries per work item
@@ -160,8 +171,8 @@ template <int BLOCK>
 struct SharedSmall {
   uint32_t v[BLOCK];
   int64_t start[BLOCK];
-  uint64_t pre[BLOCK];
-  uint64_t wave_tot[KernelShape<BLOCK>::kWaves];
+  uint32_t pre[BLOCK];
+  uint32_t wave_tot[KernelShape<BLOCK>::kWaves];
 };
 
 template <int BLOCK>
@@ -310,7 +321,7 @@ __device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, Visitor &
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
   if (lane_id() == 0 && v) atomicAdd((unsigned long long *)&sh.mf_sum, (unsigned long long)v);
   __syncthreads();
-  return sh.mf_sum;
+  return uniform(sh.mf_sum);
 }
 
 // Sum over all lanes of the workgroup of a per-lane value into sh.prof[f]
@@ -365,10 +376,12 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 }
 
 // Top-down phase A over this workgroup's equal share of the frontier.
-template <int BLOCK>
+// kProf: the per-level counters are compiled in (the profiling kernel only).
+template <int BLOCK, bool kProf>
 __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
                                 uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
+
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
   if (tid < kWave) tails_prefix<kSegments>(in_slot.tail, sh.seg_pre);
@@ -403,7 +416,7 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
       const bool is_med = deg > a.small_max && deg <= a.medium_max;
       const bool is_big = deg > a.medium_max && deg <= a.large_max;
       const bool is_huge = deg > a.large_max;
-      if (a.prof) prof_bins<BLOCK>(sh, is_small, is_med, is_big, is_huge, deg);
+      if (kProf && a.prof) prof_bins<BLOCK>(sh, is_small, is_med, is_big, is_huge, deg);
       // wave-aggregated slot reservation (ballot + mbcnt) for the bins
       const uint64_t mm = __ballot(is_med);
       if (mm) {
@@ -486,28 +499,24 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
   if (tid < kWave) tails_prefix<kSegments>(in_slot.tail, sh.seg_pre);
   __syncthreads();
   const uint32_t total = sh.seg_pre[kSegments];
-  uint32_t v = 0;
+  uint32_t v = 0, deg = 0;
   int64_t start = 0;
-  uint64_t deg = 0;
   if (tid < total) {
     uint32_t j = 0;
 #pragma unroll
     for (int k = 1; k < kSegments; ++k) j += sh.seg_pre[k] <= tid;
     v = in_seg[(size_t)j * a.n + (tid - sh.seg_pre[j])];
     start = a.off[v];
-    deg = (uint64_t)(a.off[v + 1] - start);
+    deg = (uint32_t)(a.off[v + 1] - start);
   }
-  // block-wide exclusive prefix of the degrees (64-bit)
-  uint64_t incl = deg;
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint64_t y = __shfl_up(incl, d);
-    if (lane >= (uint32_t)d) incl += y;
-  }
+  // block-wide exclusive prefix of the degrees (the caller guarantees that
+  // the frontier's out-degree sum fits 32 bits)
+  const uint32_t incl = wave_inclusive_scan(deg);
   if (lane == kWave - 1) sh.u.f.wave_tot[wave] = incl;
   __syncthreads();
-  uint64_t before = 0, T = 0;
+  uint32_t before = 0, T = 0;
   for (int i = 0; i < W; ++i) {
-    const uint64_t t = sh.u.f.wave_tot[i];
+    const uint32_t t = (uint32_t)sh.u.f.wave_tot[i];
     if ((uint32_t)i < wave) before += t;
     T += t;
   }
@@ -515,14 +524,14 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
   sh.u.f.start[tid] = start;
   sh.u.f.pre[tid] = before + incl - deg;
   __syncthreads();
-  const uint64_t lo = T * gwave / nwaves, hi = T * (gwave + 1) / nwaves;
+  const uint32_t lo = (uint32_t)((uint64_t)T * gwave / nwaves), hi = (uint32_t)((uint64_t)T * (gwave + 1) / nwaves);
   const uint32_t cnt = total ? total : 1;
-  for (uint64_t e0 = lo; e0 < hi; e0 += (uint64_t)kWave * kUnroll) {
+  for (uint32_t e0 = lo; e0 < hi; e0 += kWave * kUnroll) {
     bool act[kUnroll];
     uint32_t u[kUnroll], w[kUnroll];
 #pragma unroll
     for (int i = 0; i < kUnroll; ++i) {
-      const uint64_t e = e0 + (uint64_t)i * kWave + lane;
+      const uint32_t e = e0 + (uint32_t)i * kWave + lane;
       act[i] = e < hi;
       // last entry whose prefix is <= e: entries of degree 0 share their
       // prefix with the next one and are never chosen
@@ -543,6 +552,7 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
 template <int BLOCK>
 __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t total,
                                 uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
+
   const uint32_t lane = lane_id();
   for (uint32_t i = gwave; i < total; i += nwaves) {
     uint32_t j = 0;
@@ -570,8 +580,9 @@ __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOC
 // Frontier bitmap of a bottom-up level that follows a top-down one:
 // fb = { v : level[v] == L }, one 64-vertex group per wave step, eight
 // groups' level loads in flight at once.
-__device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t L, uint32_t *fb,
+__device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, const uint32_t *level, uint32_t L, uint32_t *fb,
                                                       uint32_t gwave, uint32_t nwaves) {
+
   constexpr int B = 8;
   const uint32_t groups = (a.n + 63) / 64;
   const uint32_t lane = lane_id();
@@ -580,7 +591,7 @@ __device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t
 #pragma unroll
     for (int i = 0; i < B; ++i) {
       const uint32_t v = (g0 + i * nwaves) * 64 + lane;
-      lv[i] = (g0 + i * nwaves < groups && v < a.n) ? a.level[v] : kInf;
+      lv[i] = (g0 + i * nwaves < groups && v < a.n) ? level[v] : kInf;
     }
 #pragma unroll
     for (int i = 0; i < B; ++i) {
@@ -603,10 +614,11 @@ __device__ __forceinline__ void build_frontier_bitmap(const BfsArgs &a, uint32_t
 // The first frontier neighbour in row order becomes the parent.  Claims are
 // collected in LDS; the wave then writes the chunk's visited words and the
 // next level's frontier-bitmap words itself (it owns them: no atomics).
-template <int BLOCK>
-__device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t L,
+template <int BLOCK, bool kProf>
+__device__ __forceinline__ void bottomup_level(const BfsArgs &a, const SearchIo &io, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t L,
                                const uint32_t *fb_cur, uint32_t *fb_nxt, uint32_t wave,
                                uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
+
   const uint32_t lane = lane_id();
   const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
   const uint4 *head = a.head;
@@ -624,7 +636,7 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
     const uint32_t cnt = (uint32_t)__popc(unv);
     const uint32_t incl = wave_inclusive_scan(cnt);
     const uint32_t total = bcast_lane(incl, 63);
-    if (a.prof && lane == 0 && total)
+    if (kProf && a.prof && lane == 0 && total)
       atomicAdd((unsigned long long *)&sh.prof[kProfBuCandidates], (unsigned long long)total);
     if (lane < kBuWords) cm[lane] = 0;
     for (uint32_t pos = incl - cnt; unv; unv &= unv - 1u) cand[pos++] = (wi << 5) + (uint32_t)(__ffs(unv) - 1);
@@ -694,7 +706,7 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
       s0 = st0 ? o0 : s0;
       e = st0 ? o1 : e;
       if (claimed) {
-        if (a.parent) a.parent[v] = par;
+        if (io.parent) io.parent[v] = par;
         vis.mf += (uint64_t)deg;
         atomicOr(&cm[(v >> 5) - w0], 1u << (v & 31u));
       }
@@ -711,7 +723,7 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, SharedAll<BLOCK
     // instruction (a few requests instead of one per claim)
     for (uint32_t p = 0; p < kBuWords / 2; ++p) {
       const uint32_t bits = cm[2 * p + (lane >> 5)];
-      if ((bits >> (lane & 31u)) & 1u) a.level[(w0 + 2 * p) * 32 + lane] = L + 1;
+      if ((bits >> (lane & 31u)) & 1u) io.level[(w0 + 2 * p) * 32 + lane] = L + 1;
     }
     wave_sync();
   }
@@ -734,7 +746,7 @@ __device__ __forceinline__ uint64_t solo_sync(SharedAll<BLOCK> &sh, uint32_t cla
   if (threadIdx.x == 0) sh.bcast[0] = claims;
   __syncthreads();
   *mf_out = mf;
-  const uint64_t n = sh.bcast[0];
+  const uint64_t n = uniform(sh.bcast[0]);
   __syncthreads();
   return n;
 }
@@ -787,14 +799,14 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
     }
   }
   __syncthreads();
-  L = (uint32_t)st[0];
-  n_cur = st[1];
-  n_prev = st[2];
-  m_f = st[3];
-  m_seen = st[4];
-  reached = st[5];
-  bu_levels = (uint32_t)st[6];
-  const bool done = st[7] != 0;
+  L = uniform((uint32_t)st[0]);
+  n_cur = uniform(st[1]);
+  n_prev = uniform(st[2]);
+  m_f = uniform(st[3]);
+  m_seen = uniform(st[4]);
+  reached = uniform(st[5]);
+  bu_levels = uniform((uint32_t)st[6]);
+  const bool done = uniform(st[7]) != 0;
   __syncthreads();
   return done;
 }
@@ -808,33 +820,33 @@ __device__ __forceinline__ bool solo_wait(Control *ctl, SharedAll<BLOCK> &sh, ui
 // has waited for its stores they are visible to every XCD without an L2
 // write-back.
 template <bool kThrough = false>
-__device__ __forceinline__ void bfs_init(const BfsArgs &a, uint32_t gtid, uint32_t gstride) {
+__device__ __forceinline__ void bfs_init(const BfsArgs &a, const SearchIo &io, uint32_t gtid, uint32_t gstride) {
   auto put = [](uint32_t *p, uint32_t x) {
     if (kThrough) __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else *p = x;
   };
   for (uint32_t i = gtid; i < a.n; i += gstride) {
-    put(&a.level[i], (i == a.source) ? 0u : kInf);
-    if (a.parent) put(&a.parent[i], (i == a.source) ? a.source : kInf);
+    put(&io.level[i], (i == io.source) ? 0u : kInf);
+    if (io.parent) put(&io.parent[i], (i == io.source) ? io.source : kInf);
   }
   for (uint32_t w = gtid; w < a.n_words; w += gstride) {
-    const uint32_t init = ((w == (a.source >> 5)) ? (1u << (a.source & 31u)) : 0u) | (a.iso ? a.iso[w] : 0u);
+    const uint32_t init = ((w == (io.source >> 5)) ? (1u << (io.source & 31u)) : 0u) | (a.iso ? a.iso[w] : 0u);
     put(&a.vis[w], init);
     put(&a.vis_snap[w], init);
   }
   const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
   if (a.bottom_up)
     for (size_t w = gtid; w < 2 * fbs; w += gstride) put(&a.fb[w], 0u);
+  // the four level slots' tails and item tails, one word per thread; the
+  // level-0 frontier is the source alone in segment 0 of slot 3
+  if (gtid < 4 * 2 * kSegments) {
+    const uint32_t s = gtid / (2 * kSegments), k = gtid % (2 * kSegments);
+    uint32_t *w = k < (uint32_t)kSegments ? &a.ctl->slot[s].tail[k] : &a.ctl->slot[s].item_tail[k - kSegments];
+    put(w, (s == 3 && k == 0) ? 1u : 0u);
+  }
   if (gtid == 0) {
-    Control *ctl = a.ctl;
-    put(&a.qseg[0][0], a.source);
-    for (int s = 0; s < 4; ++s)
-      for (int k = 0; k < kSegments; ++k) {
-        put(&ctl->slot[s].tail[k], 0u);
-        put(&ctl->slot[s].item_tail[k], 0u);
-      }
-    put(&ctl->slot[3].tail[0], 1u);
-    put(&ctl->err, 0u);
+    put(&a.qseg[0][0], io.source);
+    put(&a.ctl->err, 0u);
   }
 }
 
@@ -854,36 +866,38 @@ struct GridBfsMode {
 // One BFS over the whole grid (every workgroup must call it with the same
 // arguments): initialisation, then the level loop until a level claims no
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
-template <int BLOCK>
-__device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh, Team &team,
+// kProf: per-level counters into a.prof (compiled into the profiling kernel
+// only, so the search kernels carry none of their registers).
+template <int BLOCK, bool kProf = false>
+__device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, SharedAll<BLOCK> &sh, Team &team,
                                          GridBfsMode mode = GridBfsMode{false, false, 0}) {
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / kWave;
-  const uint32_t trank = sh.team_rank, tsize = sh.team_size;
+  const uint32_t wave = uniform(tid / kWave);  // wave-uniform: a scalar register
+  const uint32_t trank = uniform(sh.team_rank), tsize = uniform(sh.team_size);
   const uint32_t gtid = trank * BLOCK + tid;
   const uint32_t gstride = tsize * BLOCK;
   const uint32_t gwave = trank * S::kWaves + wave;
   const uint32_t nwaves = tsize * S::kWaves;
   BarrierState &bst = team.bst;
   Control *ctl = a.ctl;
-  if (tid < kProfFields) sh.prof[tid] = 0;
+  if (kProf && tid < kProfFields) sh.prof[tid] = 0;
 
-  if (a.result && gtid == 0) a.result[4] = now_ns();
+  if (io.result && gtid == 0) io.result[4] = now_ns();
   uint64_t solo_next = 0;  // tag of the next hand-back of solo levels (thread 0)
   if (mode.pre_init) {
     if (tid == 0) solo_next = (mode.solo_seen & ~kSoloDone) + 1;
   } else {
     if (tid == 0) solo_next = (ld_relaxed(&ctl->solo_rel.word) & ~kSoloDone) + 1;
-    bfs_init(a, gtid, gstride);
+    bfs_init(a, io, gtid, gstride);
     grid_barrier(&ctl->bar, bst, 0, sh.bcast);
   }
   const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
 
   Visitor vis;
   vis.col = a.col;
-  vis.level = a.level;
-  vis.parent = a.parent;
+  vis.level = io.level;
+  vis.parent = io.parent;
   vis.vis = a.vis;
   vis.snap = a.vis;
   vis.probe_live = false;
@@ -903,7 +917,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
   uint64_t n_cur = 1, n_prev = 0, reached = 1;
   // m_f: out-degree sum of the current frontier; m_seen: of every vertex
   // placed in a frontier so far (m - m_seen = edges left to explore)
-  uint64_t m_f = (uint64_t)(a.off[a.source + 1] - a.off[a.source]);
+  uint64_t m_f = (uint64_t)(a.off[io.source + 1] - a.off[io.source]);
   uint64_t m_seen = m_f;
   uint32_t bu_levels = 0;
   bool bottom_up = false;
@@ -968,7 +982,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
     uint64_t n_next;
     uint32_t hc = 0;
     uint64_t t_start = 0, busy = 0;
-    if (a.prof && tid == 0) t_start = now_ns();
+    if (kProf && a.prof && tid == 0) t_start = now_ns();
     if (!bottom_up) {
       bool fb_track = false;
       vis.fb_mark = nullptr;
@@ -997,13 +1011,13 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the
       // team in one pass; otherwise degree bins
-      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges)
+      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull)
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
       else
-        topdown_phase_a<BLOCK>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
+        topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
                                edges);
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
-      if (a.prof && tid == 0) busy += now_ns() - t_start;
+      if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
       uint64_t mf1 = 0;
       if (solo) {
         n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
@@ -1013,14 +1027,14 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       m_f = mf1;
       if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
       __syncthreads();
-      hc = sh.item_pre[kSegments];
+      hc = uniform(sh.item_pre[kSegments]);
       if (hc) {
         const uint32_t before = vis.claims;
         uint64_t th = 0;
-        if (a.prof && tid == 0) th = now_ns();
+        if (kProf && a.prof && tid == 0) th = now_ns();
         topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
-        if (a.prof && tid == 0) busy += now_ns() - th;
+        if (kProf && a.prof && tid == 0) busy += now_ns() - th;
         uint64_t mf2 = 0;
         if (solo)
           n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
@@ -1033,23 +1047,23 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
       uint32_t *fb_cur = a.fb + (L & 1) * fbs;
       uint32_t *fb_nxt = a.fb + ((L + 1) & 1) * fbs;
       if (!fb_valid) {
-        build_frontier_bitmap(a, L, fb_cur, gwave, nwaves);
+        build_frontier_bitmap(a, io.level, L, fb_cur, gwave, nwaves);
         grid_barrier(&ctl->bar, bst, 0, sh.bcast);
       }
       vis.fb_mark = nullptr;
       fb_zero = 0;
       fb_valid = true;
       uint64_t tb = 0;
-      if (a.prof && tid == 0) tb = now_ns();
-      bottomup_level<BLOCK>(a, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
+      if (kProf && a.prof && tid == 0) tb = now_ns();
+      bottomup_level<BLOCK, kProf>(a, io, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
-      if (a.prof && tid == 0) busy += now_ns() - tb;
+      if (kProf && a.prof && tid == 0) busy += now_ns() - tb;
       uint64_t mf1 = 0;
       n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
       m_f = mf1;
     }
     m_seen += m_f;
-    if (a.prof) {
+    if (kProf && a.prof) {
       prof_add<BLOCK>(sh, kProfEdges, edges);
       prof_add<BLOCK>(sh, kProfProbes, vis.probes);
       prof_add<BLOCK>(sh, kProfAtomics, vis.atomics);
@@ -1075,12 +1089,12 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, SharedAll<BLOCK> &sh,
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
         ctl->edges = m_seen;
-        if (a.result) {
-          a.result[0] = L + 1;
-          a.result[1] = reached;
-          a.result[2] = bu_levels;
-          a.result[3] = m_seen;
-          a.result[5] = now_ns();
+        if (io.result) {
+          io.result[0] = L + 1;
+          io.result[1] = reached;
+          io.result[2] = bu_levels;
+          io.result[3] = m_seen;
+          io.result[5] = now_ns();
         }
       }
       if (solo) solo_publish(ctl, solo_tag, L, n_cur, n_prev, m_f, m_seen, reached, bu_levels, true, mode.quick_end);
@@ -1129,7 +1143,7 @@ __device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_
   return sh.bcast[0] <= (uint64_t)kSegments;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool kProf = false>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
@@ -1142,7 +1156,8 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   // counted, so before any hand-back of this search.
   uint64_t solo_seen = 0;
   if (threadIdx.x == 0) solo_seen = ld_relaxed(&a.ctl->solo_rel.word);
-  bfs_init<true>(a, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
+  const SearchIo io{a.level, a.parent, a.result, a.source};
+  bfs_init<true>(a, io, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
   wait_own_stores();
   __syncthreads();
   bool abandoned = false;
@@ -1163,7 +1178,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   if (threadIdx.x == 0) l2_invalidate();
   __syncthreads();
   // the kernel ends after the search, which writes every L2 back
-  grid_bfs<BLOCK>(a, sh, team, GridBfsMode{true, true, solo_seen});
+  grid_bfs<BLOCK, kProf>(a, io, sh, team, GridBfsMode{true, true, solo_seen});
 }
 
 // Many searches of one graph in one launch, in turn: search i starts from
@@ -1171,7 +1186,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
 // parent_stride when parents are asked for) and result + i * kResultWords.
 // Each search's first grid barrier also ends the previous one.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(BfsArgs a, const uint32_t *sources, uint32_t count,
+__global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(const BfsArgs a, const uint32_t *sources, uint32_t count,
                                                             uint64_t level_stride, uint64_t parent_stride) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
@@ -1180,17 +1195,12 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(BfsArgs a, const uin
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->err = 2;
     return;
   }
-  uint32_t *const level0 = a.level;
-  uint32_t *const parent0 = a.parent;
-  uint64_t *const result0 = a.result;
   for (uint32_t i = 0; i < count; ++i) {
-    a.source = sources[i];
-    a.level = level0 + i * level_stride;
-    a.parent = parent0 ? parent0 + i * parent_stride : nullptr;
-    a.result = result0 + (size_t)i * kResultWords;
+    const SearchIo io{a.level + i * level_stride, a.parent ? a.parent + i * parent_stride : nullptr,
+                      a.result + (size_t)i * kResultWords, uniform(sources[i])};
     // the last search ends with the kernel; the others are followed by the
     // next one's initialisation, which needs their stores written back
-    grid_bfs<BLOCK>(a, sh, team, GridBfsMode{false, i + 1 == count, 0});
+    grid_bfs<BLOCK>(a, io, sh, team, GridBfsMode{false, i + 1 == count, 0});
   }
 }
 

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.bu_exit = d.beta ? g.n / d.beta : 0;
     a.m = g.m;
     // the system-scope barrier right after writes every L2 back
-    grid_bfs<BLOCK>(a, sh, team, GridBfsMode{false, true, 0});
+    grid_bfs<BLOCK>(a, SearchIo{a.level, a.parent, a.result, a.source}, sh, team, GridBfsMode{false, true, 0});
     // outputs reach memory the host's copies read before the result appears
     grid_barrier<true>(bar, team.bst, 0, sh.bcast);
     if (sh.team_rank == 0 && tid == 0) {

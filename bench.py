@@ -321,25 +321,39 @@ def run_serve(args, dfq):
     dbuf.free()
     del dgs
 
-    with dfq.Server(graphs, xcds_per_team=1) as srv:
-        return serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full)
+    out = None
+    for groups in ([True, False] if args.serve_groups else [False]):
+        shapes = args.serve_group_shapes if groups else args.serve_shapes
+        with dfq.Server(graphs, xcds_per_team=shapes[0][0], block_size=shapes[0][1], groups=groups) as srv:
+            r = serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full, shapes, groups)
+        log(f"serve {'groups' if groups else 'single'}: {r['value']} q/s, {r['speedup_vs_plain']}x, "
+            f"p99 {r['poisson']['p99_ms']} ms")
+        if out is None:
+            out = r
+        else:
+            out["single_query_tasks"] = {k: r[k] for k in ("value", "speedup_vs_plain", "results_exact_vs_plain",
+                                                           "gteps_served", "poisson", "shape_sweep_qps")}
+            out["single_query_tasks"]["config"] = {k: r["config"][k] for k in ("xcds_per_team", "teams",
+                                                                              "block_size")}
+    return out
 
 
-def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full):
+def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, k_full, shapes, groups):
     # team width: the closed-loop rate of the plain sample at each shape
     sweep = {}
     exact = True
-    for xpt in args.serve_shapes:
-        srv.set_shape(xpt)
+    for shape in shapes:
+        srv.set_shape(*shape)
         srv.start()
         recs, _ = srv.replay(pg, ps, np.zeros(len(pg)))
-        sweep[xpt] = len(pg) / (recs["done_us"].max() * 1e-6)
+        sweep[shape] = len(pg) / (recs["done_us"].max() * 1e-6)
         exact &= bool(np.array_equal(recs["reached"], plain[:, 0]) and
                       np.array_equal(recs["depth"].astype(np.uint64), plain[:, 1]))
         srv.stop()
-        log(f"  xcds/team {xpt}: {sweep[xpt]:.1f} q/s closed loop on the plain sample")
+        log(f"  {'groups' if groups else 'single'}, xcds/team {shape[0]}, block {shape[1]}: "
+            f"{sweep[shape]:.1f} q/s closed loop on the plain sample")
     best = max(sweep, key=sweep.get)
-    srv.set_shape(best)
+    srv.set_shape(*best)
     srv.start()
     info_s = srv.info()
     # full level arrays of a few queries, served in the mix, against the plain engine
@@ -372,6 +386,8 @@ def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, 
                                            "service_ms_mean": round(float(recs2["service_us"][m].mean()) / 1e3, 3)}
     out = {
         "metric": "served BFS queries/sec vs back-to-back plain launches, with p50/p99 latency",
+        "tasks": "group tasks: queries of one graph waiting together share one bit-parallel "
+                 "multi-source traversal (up to 64)" if groups else "one search per team task",
         "value": round(sat_qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": 1,
         "warmup": len(args.serve_shapes), "ms_per_step": round(sat_s * 1e3, 1), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
@@ -380,7 +396,7 @@ def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, 
                                "degree > 0",
                    "queries": len(gi), "vertices_total": int(sum(g.n for g in graphs)),
                    "csr_entries_total": int(sum(g.m for g in graphs)),
-                   "xcds_per_team": best, "teams": info_s["teams"], "block_size": info_s["block_size"],
+                   "xcds_per_team": best[0], "teams": info_s["teams"], "block_size": info_s["block_size"],
                    "arena_bytes": info_s["arena_bytes"]},
         "plain_queries_per_s": round(plain_qps, 1),
         "plain_sample": len(pg),
@@ -393,7 +409,7 @@ def serve_on(args, srv, graphs, scales, pg, ps, plain, plain_qps, plain_levels, 
                     "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
                     "admission_wait_p99_ms": round(float(np.percentile(wait, 99)), 3),
                     "by_graph": lat_by_scale},
-        "shape_sweep_qps": {str(k): round(v, 1) for k, v in sweep.items()},
+        "shape_sweep_qps": {f"{k[0]}x{k[1]}": round(v, 1) for k, v in sweep.items()},
         "results_exact_vs_plain": exact,
     }
     return out
@@ -428,10 +444,16 @@ def main():
                     help="Poisson rate as a fraction of the saturated rate")
     ap.add_argument("--serve-shapes", type=int, nargs="+", default=[1, 2, 4],
                     help="XCDs per team to compare (the fastest serves the trace)")
+    ap.add_argument("--serve-group-shapes", type=str, nargs="+", default=["1x512", "1x1024", "2x1024", "8x1024"],
+                    help="XCDs-per-team x workgroup size for group-mode serving")
+    ap.add_argument("--no-serve-groups", dest="serve_groups", action="store_false",
+                    help="serve single-query tasks only")
     ap.add_argument("--then-copy", type=int, default=0, metavar="BYTES",
                     help="after the benchmark, run the streaming copy of BYTES once (so one "
                          "counter pass covers the BFS and its calibration)")
     args = ap.parse_args()
+    args.serve_shapes = [(x, 1024) for x in args.serve_shapes]
+    args.serve_group_shapes = [tuple(int(v) for v in x.split("x")) for x in args.serve_group_shapes]
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs the search on one "
               f"GPU (multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -546,7 +568,10 @@ def main():
         out["batch"]["config"] = b["config"]
         sv = run_serve(args, dfq)
         out["serve"] = {k: sv[k] for k in ("value", "unit", "plain_queries_per_s", "speedup_vs_plain",
-                                           "results_exact_vs_plain", "gteps_served")}
+                                           "results_exact_vs_plain", "gteps_served", "tasks")}
+        if "single_query_tasks" in sv:
+            out["serve"]["single_query_tasks"] = {k: sv["single_query_tasks"][k]
+                                                  for k in ("value", "speedup_vs_plain", "results_exact_vs_plain")}
         out["serve"]["poisson"] = {k: sv["poisson"][k] for k in ("rate_qps", "queries", "achieved_qps",
                                                                  "p50_ms", "p99_ms")}
         out["serve"]["config"] = sv["config"]

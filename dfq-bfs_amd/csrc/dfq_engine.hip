@@ -18,6 +18,7 @@
 
 #include "dfq_bfs.h"
 #include "dfq_kernels.hpp"
+#include "dfq_msbfs.hpp"
 #include "host_common.hpp"
 
 using namespace dfq;
@@ -71,6 +72,21 @@ struct dfq_graph {
   uint32_t *d_qseg[2] = {nullptr, nullptr};  // kSegments x n entries each
   uint64_t *d_prof = nullptr;
   uint32_t prof_levels = 0;
+  // multi-source searches (allocated at the first one): seen, two frontier
+  // mask arrays and the level planes; three queue buffers; per-search
+  // counters, depths and sources
+  uint64_t *d_ms = nullptr;
+  uint32_t *d_msq = nullptr;
+  uint64_t *d_mscnt = nullptr;
+  uint64_t *d_msprof = nullptr;
+  uint32_t msprof_levels = 0;
+  int grid_ms = 0;
+  // degree-ranked copy of the graph for multi-source searches (built at the
+  // first one): offsets, columns as ranks (rows sorted), vertex -> rank
+  int64_t *d_off_r = nullptr;
+  uint32_t *d_col_r = nullptr;
+  uint32_t *d_o2r = nullptr;
+  uint32_t nz = 0;  // vertices with entries (ranks [0, nz))
   Control *d_ctl = nullptr;
   // back-to-back searches: events between launches, per-search results,
   // the sources of a one-launch sequence
@@ -98,6 +114,8 @@ int validate_config(const dfq_config &c) {
 }
 
 static_assert(DFQ_PROFILE_FIELDS == kProfFields, "header and kernel disagree on the profile row");
+static_assert(DFQ_MULTI_MAX_SOURCES == kMsMaxSources, "header and kernel disagree on the multi-source width");
+static_assert(DFQ_MULTI_PROFILE_FIELDS == kMsProfFields, "header and kernel disagree on the profile row");
 
 // prof: the instantiation with the per-level counters compiled in (used by
 // dfq_bfs_profile only; the search kernels carry none of that code).
@@ -161,6 +179,13 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_head);
   (void)hipFree(g->d_items);
   (void)hipFree(g->d_prof);
+  (void)hipFree(g->d_ms);
+  (void)hipFree(g->d_msq);
+  (void)hipFree(g->d_mscnt);
+  (void)hipFree(g->d_msprof);
+  (void)hipFree(g->d_off_r);
+  (void)hipFree(g->d_col_r);
+  (void)hipFree(g->d_o2r);
   (void)hipFree(g->d_ctl);
   (void)hipFree(g->d_res);
   (void)hipFree(g->d_src);
@@ -580,6 +605,168 @@ int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources, ui
   }
   return DFQ_OK;
 }
+
+namespace {
+int run_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level, uint64_t level_stride,
+              dfq_stats *stats, double *launch_ms, uint64_t *prof_rows, uint32_t prof_levels);
+}
+
+int dfq_bfs_device_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level,
+                         uint64_t level_stride, dfq_stats *stats, double *launch_ms) {
+  return run_multi(g, count, sources, d_level, level_stride, stats, launch_ms, nullptr, 0);
+}
+
+int dfq_bfs_multi_profile(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level,
+                          uint64_t level_stride, uint64_t *rows, uint32_t max_levels, dfq_stats *stats) {
+  if (!rows || max_levels == 0) return DFQ_ERR_INVALID_ARG;
+  return run_multi(g, count, sources, d_level, level_stride, stats, nullptr, rows, max_levels);
+}
+
+}  // extern "C"
+
+namespace {
+int run_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d_level, uint64_t level_stride,
+              dfq_stats *stats, double *launch_ms, uint64_t *prof_rows, uint32_t prof_levels) {
+  if (!g || !d_level || !sources || !stats || count == 0 || count > kMsMaxSources) return DFQ_ERR_INVALID_ARG;
+  if (level_stride < g->n || level_stride > UINT64_MAX / count) return DFQ_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < count; ++i)
+    if (sources[i] >= g->n) return DFQ_ERR_RANGE;
+  HIP_TRY(hipSetDevice(g->device));
+  const size_t n = g->n;
+  if (!g->d_ms) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&dfq_msbfs_kernel<1024>, 1024, 0));
+    if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, g->device));
+    hipError_t e = hipMalloc(&g->d_ms, sizeof(uint64_t) * n * (3 + kMsPlanes));
+    if (e == hipSuccess) e = hipMalloc(&g->d_msq, sizeof(uint32_t) * n * 3 * kSegments);
+    if (e == hipSuccess) e = hipMalloc(&g->d_mscnt, sizeof(uint64_t) * (2 * kMsMaxSources) + 2 * sizeof(uint32_t) * kMsMaxSources);
+    if (e != hipSuccess) {
+      (void)hipFree(g->d_ms);
+      (void)hipFree(g->d_msq);
+      (void)hipFree(g->d_mscnt);
+      g->d_ms = nullptr;
+      g->d_msq = nullptr;
+      g->d_mscnt = nullptr;
+      last_hip_error() = e;
+      return DFQ_ERR_HIP;
+    }
+    g->grid_ms = prop.multiProcessorCount * per_cu;
+  }
+  if (!g->d_off_r) {
+    // the degree-ranked copy, built on the host from the resident CSR
+    std::vector<int64_t> off, off_r;
+    std::vector<uint32_t> col, col_r, o2r, r2o;
+    try {
+      off.resize(n + 1);
+      off_r.resize(n + 1);
+      col.resize(g->m ? g->m : 1);
+      col_r.resize(g->m ? g->m : 1);
+      o2r.resize(n);
+      r2o.resize(n);
+    } catch (...) {
+      return DFQ_ERR_NOMEM;
+    }
+    HIP_TRY(hipMemcpy(off.data(), g->d_off, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    if (g->m) HIP_TRY(hipMemcpy(col.data(), g->d_col, sizeof(uint32_t) * g->m, hipMemcpyDeviceToHost));
+    rank_relabel(off.data(), col.data(), g->n, o2r.data(), r2o.data(), off_r.data(), col_r.data());
+    uint32_t nz = g->n;
+    while (nz > 0 && off_r[nz - 1] == off_r[nz]) --nz;
+    g->nz = nz;
+    hipError_t e = hipMalloc(&g->d_off_r, sizeof(int64_t) * (n + 1));
+    // +4 entries: bottom-up levels read columns as aligned 16-byte blocks
+    if (e == hipSuccess) e = hipMalloc(&g->d_col_r, sizeof(uint32_t) * (g->m + 4));
+    if (e == hipSuccess) e = hipMalloc(&g->d_o2r, sizeof(uint32_t) * n);
+    if (e == hipSuccess) e = hipMemcpy(g->d_off_r, off_r.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g->m) e = hipMemcpy(g->d_col_r, col_r.data(), sizeof(uint32_t) * g->m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->d_o2r, o2r.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(g->d_off_r);
+      (void)hipFree(g->d_col_r);
+      (void)hipFree(g->d_o2r);
+      g->d_off_r = nullptr;
+      g->d_col_r = nullptr;
+      g->d_o2r = nullptr;
+      last_hip_error() = e;
+      return DFQ_ERR_HIP;
+    }
+  }
+  uint32_t *d_depth = (uint32_t *)(g->d_mscnt + 2 * kMsMaxSources);
+  uint32_t *d_src = d_depth + kMsMaxSources;
+  HIP_TRY(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * count, hipMemcpyHostToDevice, g->stream));
+  MsArgs a;
+  memset(&a, 0, sizeof(a));
+  a.off = g->d_off_r;
+  a.col = g->d_col_r;
+  a.col_bu = nullptr;  // the ranked rows are already highest degree first
+  a.o2r = g->d_o2r;
+  a.nz = g->nz;
+  a.n = g->n;
+  a.count = count;
+  a.m = g->m;
+  a.sources = d_src;
+  a.level = d_level;
+  a.level_stride = level_stride;
+  a.seen = g->d_ms;
+  a.f = g->d_ms + n;
+  a.planes = g->d_ms + 3 * n;
+  a.qb = g->d_msq;
+  a.segs = kSegments;
+  a.items = g->d_items;
+  a.item_cap = g->item_cap;
+  a.ctl = g->d_ctl;
+  a.counters = g->d_mscnt;
+  a.depth = d_depth;
+  a.small_max = g->cfg.small_max;
+  a.medium_max = g->cfg.medium_max;
+  a.large_max = g->cfg.large_max;
+  a.bottom_up = (g->symmetric && g->cfg.direction_opt) ? 1u : 0u;
+  a.bu_div = g->cfg.multi_bu_div ? g->cfg.multi_bu_div : kMsBuDiv;
+  if (prof_rows) {
+    if (prof_levels > g->msprof_levels) {
+      (void)hipFree(g->d_msprof);
+      g->d_msprof = nullptr;
+      g->msprof_levels = 0;
+      HIP_TRY(hipMalloc(&g->d_msprof, sizeof(uint64_t) * kMsProfFields * prof_levels));
+      g->msprof_levels = prof_levels;
+    }
+    HIP_TRY(hipMemsetAsync(g->d_msprof, 0, sizeof(uint64_t) * kMsProfFields * prof_levels, g->stream));
+    a.prof = g->d_msprof;
+    a.prof_levels = prof_levels;
+  }
+  void *args[] = {&a};
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  HIP_TRY(hipLaunchCooperativeKernel((const void *)&dfq_msbfs_kernel<1024>, dim3(g->grid_ms), dim3(1024), args, 0,
+                                     g->stream));
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  Control c;
+  HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  if (c.err == kMsErrDepth) return DFQ_ERR_CAPACITY;
+  if (c.err) return DFQ_ERR_HIP;
+  uint64_t cnt[2 * kMsMaxSources];
+  uint32_t dep[kMsMaxSources];
+  HIP_TRY(hipMemcpy(cnt, g->d_mscnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dep, d_depth, sizeof(dep), hipMemcpyDeviceToHost));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  for (uint32_t i = 0; i < count; ++i) {
+    stats[i].kernel_ms = ms;
+    stats[i].depth = dep[i];
+    stats[i].reached = cnt[i];
+    stats[i].grid_blocks = (uint32_t)g->grid_ms;
+    stats[i].block_size = 1024;
+    stats[i].bottom_up_levels = c.bu_levels;
+  }
+  if (launch_ms) *launch_ms = ms;
+  if (prof_rows)
+    HIP_TRY(hipMemcpy(prof_rows, g->d_msprof, sizeof(uint64_t) * kMsProfFields * prof_levels, hipMemcpyDeviceToHost));
+  return DFQ_OK;
+}
+}  // namespace
+
+extern "C" {
 
 int dfq_bfs_profile(dfq_graph *g, uint32_t source, uint64_t *rows, uint32_t max_levels, dfq_stats *stats) {
   if (!g || !rows || max_levels == 0) return DFQ_ERR_INVALID_ARG;

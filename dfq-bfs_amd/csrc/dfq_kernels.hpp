@@ -45,7 +45,9 @@ namespace dfq {
 struct alignas(256) LevelSlot {
   uint32_t tail[kSegments];       // entries in each segment of level L's output
   uint32_t item_tail[kSegments];  // work items of level L's grid phase, per segment
-  uint32_t pad[64 - 2 * kSegments];
+  uint64_t active;                // multi-source searches: the searches with a frontier at level L
+  uint32_t chunk_next;            // multi-source bottom-up levels: next vertex chunk to claim
+  uint32_t pad[64 - 2 * kSegments - 3];
 };
 
 // Huge-degree frontier vertices are cut into items of kItem CSR entries:
@@ -199,8 +201,8 @@ struct SharedAll {
 };
 
 // Wave-uniform gather of the edges of up to 64 lane-owned vertices.
-template <int BLOCK>
-__device__ __forceinline__ void gather_small(Visitor &vis, uint32_t v, int64_t start, uint32_t deg,
+template <int BLOCK, class V>
+__device__ __forceinline__ void gather_small(V &vis, uint32_t v, int64_t start, uint32_t deg,
                                              uint32_t *g_excl, uint32_t *g_v, int64_t *g_s) {
   const uint32_t lane = lane_id();
   const uint32_t incl = wave_inclusive_scan(deg);
@@ -226,14 +228,14 @@ __device__ __forceinline__ void gather_small(Visitor &vis, uint32_t v, int64_t s
       u[i] = g_v[j];
       w[i] = act[i] ? vis.col[g_s[j] + (int64_t)(e - g_excl[j])] : 0u;
     }
-    vis.visit_batch<kUnroll, true>(act, u, w);
+    vis.template visit_batch<kUnroll, true>(act, u, w);
   }
   wave_sync();
 }
 
 // Expands the medium LDS queue (one wavefront per vertex).
-template <int BLOCK>
-__device__ __forceinline__ void expand_medium(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t cnt,
+template <int BLOCK, class V>
+__device__ __forceinline__ void expand_medium(V &vis, SharedAll<BLOCK> &sh, uint32_t cnt,
                                               uint32_t wave) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
   const uint32_t lane = lane_id();
@@ -251,14 +253,14 @@ __device__ __forceinline__ void expand_medium(Visitor &vis, SharedAll<BLOCK> &sh
         uu[i] = u;
         w[i] = act[i] ? vis.col[s + k] : 0u;
       }
-      vis.visit_batch<kUnroll, true>(act, uu, w);
+      vis.template visit_batch<kUnroll, true>(act, uu, w);
     }
   }
 }
 
 // Expands the large LDS queue (the whole workgroup per vertex).
-template <int BLOCK>
-__device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t cnt) {
+template <int BLOCK, class V>
+__device__ __forceinline__ void expand_large(V &vis, SharedAll<BLOCK> &sh, uint32_t cnt) {
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t u = sh.u.a.big_v[j];
     const uint32_t d = sh.u.a.big_d[j];
@@ -273,7 +275,7 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
         uu[i] = u;
         w[i] = act[i] ? vis.col[s + k] : 0u;
       }
-      vis.visit_batch<kUnroll, true>(act, uu, w);
+      vis.template visit_batch<kUnroll, true>(act, uu, w);
     }
   }
 }
@@ -282,8 +284,8 @@ __device__ __forceinline__ void expand_large(Visitor &vis, SharedAll<BLOCK> &sh,
 // workgroup, and sums the wave-uniform `v` over the workgroup (thread 0
 // gets the total in the return value).  Must be reached by all threads of
 // the workgroup.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t flush_block(Visitor &vis, SharedAll<BLOCK> &sh, uint32_t wave, uint32_t v = 0) {
+template <int BLOCK, class V>
+__device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, uint32_t wave, uint32_t v = 0) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
   if (lane_id() == 0) {
     sh.stage_n[wave] = vis.out_n;
@@ -312,8 +314,8 @@ __device__ __forceinline__ uint32_t flush_block(Visitor &vis, SharedAll<BLOCK> &
 
 // Workgroup sum of the per-lane out-degree sums of claimed vertices (thread
 // 0 gets it); resets the lanes' counters.  Ends with a __syncthreads.
-template <int BLOCK>
-__device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, Visitor &vis) {
+template <int BLOCK, class V>
+__device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, V &vis) {
   if (threadIdx.x == 0) sh.mf_sum = 0;
   __syncthreads();
   uint64_t v = vis.mf;
@@ -377,8 +379,8 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 
 // Top-down phase A over this workgroup's equal share of the frontier.
 // kProf: the per-level counters are compiled in (the profiling kernel only).
-template <int BLOCK, bool kProf>
-__device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis,
+template <int BLOCK, bool kProf, class V>
+__device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
                                 uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
 
@@ -490,8 +492,8 @@ constexpr uint64_t kSmallFrontierEdges = 8192;
 // concatenated edge list evenly (edge-balanced, no bins, no work items, so
 // the level needs one barrier).  An edge's vertex is found by a binary
 // search of the degree prefix in LDS.
-template <int BLOCK>
-__device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, const uint32_t *in_seg,
+template <int BLOCK, class V>
+__device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis, const uint32_t *in_seg,
                               const LevelSlot &in_slot, uint32_t wave, uint32_t gwave, uint32_t nwaves,
                               uint64_t &edges) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
@@ -542,15 +544,15 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
       w[i] = act[i] ? a.col[sh.u.f.start[j] + (int64_t)(e - sh.u.f.pre[j])] : 0u;
       if (act[i]) ++edges;
     }
-    vis.visit_batch<kUnroll, false>(act, u, w);
+    vis.template visit_batch<kUnroll, false>(act, u, w);
   }
   __syncthreads();
 }
 
 // Phase H: the level's work items, round-robin over every wavefront of the
 // grid.  `total` items located through sh.item_pre.
-template <int BLOCK>
-__device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, Visitor &vis, uint32_t total,
+template <int BLOCK, class V>
+__device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis, uint32_t total,
                                 uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
 
   const uint32_t lane = lane_id();
@@ -573,7 +575,7 @@ __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOC
       uu[k] = u;
       w[k] = act[k] ? a.col[s + e] : 0u;
     }
-    vis.visit_batch<kUnroll, false>(act, uu, w);
+    vis.template visit_batch<kUnroll, false>(act, uu, w);
   }
 }
 

@@ -97,17 +97,32 @@ class Arena {
 // result; admitted queries are tracked apart, so a finished query the caller
 // has not released holds arena room but no slot.
 struct RingSlot {
-  uint64_t ticket = 0;
+  uint64_t ticket = 0;  // ring ticket of the task in this slot
   bool busy = false;
-  bool copy = false;  // an internal copy task
+  bool copy = false;    // an internal copy task
+  uint64_t qid0 = 0;    // its queries: qid0 .. qid0 + count - 1
+  uint32_t count = 0;
 };
 
+// One admitted query.  Tickets handed to callers are query ids; a query
+// runs in the ring task `ring_ticket`, alone or as member `member` of a
+// group task whose arena range it shares (`group`).
 struct Query {
-  uint64_t work = 0, bytes = 0;  // arena range
+  uint64_t work = 0, bytes = 0;  // arena range (the group's for members)
+  uint64_t ring_ticket = 0;
+  uint64_t group = 0;            // group id, 0 for a single query
+  uint32_t member = 0;
   uint32_t graph = 0, source = 0, flags = 0;
   uint32_t item_cap = 0;
+  uint32_t count = 1;            // searches of its task
   bool done = false;
+  int status = DFQ_OK;           // DFQ_ERR_CAPACITY: the group went past its level limit
   dfq_serve_result result{};
+};
+
+struct GroupRange {
+  uint64_t work = 0, bytes = 0;
+  uint32_t held = 0;  // members not yet released
 };
 
 constexpr uint32_t kDrainMs = 120000;  // stop: wait for published queries
@@ -139,6 +154,14 @@ struct dfq_server {
   ServeArgs args{};
   uint8_t *h_stage = nullptr;  // pinned, host-mapped staging of copy tasks
   uint8_t *d_stage = nullptr;  // its device address
+  uint32_t *h_gsrc = nullptr;  // pinned, host-mapped [ring][64] group sources
+  ServeGroupResult *h_gres = nullptr;  // pinned, host-mapped [ring][64] group results
+  int64_t *d_off_r = nullptr;  // degree-ranked copies (group mode), indexed like d_off
+  uint32_t *d_col_r = nullptr;
+  uint32_t *d_o2r = nullptr;
+  uint64_t next_qid = 1;       // query ids (tickets); 0 is never one
+  uint64_t next_group = 1;
+  std::unordered_map<uint64_t, GroupRange> groups_held;
   dfq_serve_config cfg{};
   int grid = 0;
   bool running = false;
@@ -152,8 +175,20 @@ struct dfq_server {
 
 namespace {
 
-const void *serve_kernel_for(uint32_t block) {
-  return block == 512 ? (const void *)&dfq_serve_kernel<512> : (const void *)&dfq_serve_kernel<1024>;
+// Work items a group task's levels may create (top-down rows above
+// large_max, bottom-up rows above kMsBuRow), per segment.
+uint32_t group_item_cap(const ServeGraph &g, const dfq_query_plan &p) {
+  return serve_item_cap(g.n, g.m, std::min<uint32_t>(p.large_max, kMsBuRow));
+}
+
+uint64_t group_footprint(const ServeGraph &g, const dfq_query_plan &p, uint32_t xpt, uint32_t count) {
+  return serve_group_layout(g.n, count, group_item_cap(g, p), serve_segments(xpt)).bytes;
+}
+
+const void *serve_kernel_for(uint32_t block, bool groups) {
+  if (groups)
+    return block == 512 ? (const void *)&dfq_serve_kernel<512, true> : (const void *)&dfq_serve_kernel<1024, true>;
+  return block == 512 ? (const void *)&dfq_serve_kernel<512, false> : (const void *)&dfq_serve_kernel<1024, false>;
 }
 
 dfq_query_plan default_serve_plan() {
@@ -189,10 +224,12 @@ bool kernel_ended(dfq_server *s) { return hipStreamQuery(s->stream) != hipErrorN
 
 void free_all(dfq_server *s) {
   if (s->wedged) return;  // a launch may still read this memory
-  void *dev[] = {s->d_off, s->d_col, s->d_col_bu, s->d_head, s->d_graphs, s->d_iso, s->d_ctl, s->d_arena};
+  void *dev[] = {s->d_off,  s->d_col, s->d_col_bu, s->d_head,  s->d_graphs, s->d_iso,
+                 s->d_ctl,  s->d_arena, s->d_off_r, s->d_col_r, s->d_o2r};
   for (void *p : dev) (void)hipFree(p);
   if (s->h_block) (void)hipHostFree(s->h_block);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->h_gres) (void)hipHostFree(s->h_gres);
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -204,16 +241,28 @@ bool collect(dfq_server *s, uint32_t idx) {
   const ServeResult *r = &s->h_res[idx];
   if (__atomic_load_n(&r->done, __ATOMIC_ACQUIRE) != rs.ticket - s->base + 1) return false;
   if (!rs.copy) {
-    Query &q = s->queries[rs.ticket];
-    q.result.graph = q.graph;
-    q.result.source = q.source;
-    q.result.depth = r->depth;
-    q.result.bottom_up_levels = r->bu_levels;
-    q.result.reached = r->reached;
-    q.result.edges = r->edges;
-    q.result.team = r->team;
-    q.result.service_us = (double)(r->t_end_ns - r->t_start_ns) / 1000.0;
-    q.done = true;
+    for (uint32_t i = 0; i < rs.count; ++i) {
+      auto it = s->queries.find(rs.qid0 + i);
+      if (it == s->queries.end()) continue;
+      Query &q = it->second;
+      q.result.graph = q.graph;
+      q.result.source = q.source;
+      q.result.bottom_up_levels = r->bu_levels;
+      q.result.team = r->team;
+      q.result.service_us = (double)(r->t_end_ns - r->t_start_ns) / 1000.0;
+      if (q.group) {
+        const ServeGroupResult &g = s->h_gres[(size_t)idx * kMsMaxSources + q.member];
+        q.result.depth = __atomic_load_n(&g.depth, __ATOMIC_RELAXED);
+        q.result.reached = __atomic_load_n(&g.reached, __ATOMIC_RELAXED);
+        q.result.edges = __atomic_load_n(&g.edges, __ATOMIC_RELAXED);
+        if (r->err) q.status = DFQ_ERR_CAPACITY;
+      } else {
+        q.result.depth = r->depth;
+        q.result.reached = r->reached;
+        q.result.edges = r->edges;
+      }
+      q.done = true;
+    }
   }
   rs.busy = false;
   rs.copy = false;
@@ -225,10 +274,26 @@ Query *find(dfq_server *s, uint64_t ticket) {
   auto it = s->queries.find(ticket);
   if (it == s->queries.end()) return nullptr;
   if (!it->second.done) {
-    const uint32_t idx = (uint32_t)(ticket % s->cfg.ring_slots);
-    if (s->ring[idx].busy && s->ring[idx].ticket == ticket) collect(s, idx);
+    const uint64_t rt = it->second.ring_ticket;
+    const uint32_t idx = (uint32_t)(rt % s->cfg.ring_slots);
+    if (s->ring[idx].busy && s->ring[idx].ticket == rt) collect(s, idx);
   }
   return &it->second;
+}
+
+// Gives a finished query's arena share back.
+void release_query(dfq_server *s, std::unordered_map<uint64_t, Query>::iterator it) {
+  Query &q = it->second;
+  if (q.group) {
+    auto g = s->groups_held.find(q.group);
+    if (g != s->groups_held.end() && --g->second.held == 0) {
+      s->arena.release(g->second.work, g->second.bytes);
+      s->groups_held.erase(g);
+    }
+  } else {
+    s->arena.release(q.work, q.bytes);
+  }
+  s->queries.erase(it);
 }
 
 bool any_running(dfq_server *s) {
@@ -237,8 +302,9 @@ bool any_running(dfq_server *s) {
   return false;
 }
 
-// Fills and publishes the next ticket's slot (which must not be busy).
-uint64_t publish(dfq_server *s, const ServeRequest &req, bool copy) {
+// Fills and publishes the next ticket's slot (which must not be busy);
+// its queries are qid0 .. qid0 + count - 1.
+uint64_t publish(dfq_server *s, const ServeRequest &req, bool copy, uint64_t qid0 = 0, uint32_t count = 0) {
   const uint64_t T = s->next_ticket++;
   const uint32_t idx = (uint32_t)(T % s->cfg.ring_slots);
   ServeRequest *r = &s->h_req[idx];
@@ -258,6 +324,8 @@ uint64_t publish(dfq_server *s, const ServeRequest &req, bool copy) {
   s->ring[idx].ticket = T;
   s->ring[idx].busy = true;
   s->ring[idx].copy = copy;
+  s->ring[idx].qid0 = qid0;
+  s->ring[idx].count = count;
   return T;
 }
 
@@ -317,6 +385,7 @@ int dfq_serve_config_default(dfq_serve_config *cfg) {
   cfg->block_size = 1024;
   cfg->xcds_per_team = 1;
   cfg->ring_slots = 1024;
+  cfg->groups = 0;
   cfg->arena_bytes = 0;
   return DFQ_OK;
 }
@@ -331,6 +400,14 @@ uint64_t dfq_serve_footprint(uint32_t n, uint64_t m, uint32_t large_max, uint32_
   return footprint_of(g, p, xcds_per_team, flags, nullptr);
 }
 
+uint64_t dfq_serve_group_footprint(uint32_t n, uint64_t m, uint32_t count, uint32_t xcds_per_team) {
+  if (count == 0 || count > kMsMaxSources) return 0;
+  ServeGraph g{};
+  g.n = n;
+  g.m = m;
+  return group_footprint(g, default_serve_plan(), xcds_per_team, count);
+}
+
 int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, const int64_t *row_offsets,
                       const uint32_t *col_idx, const dfq_serve_config *cfg_in, dfq_server **out) {
   if (!out || num_graphs == 0 || !n || !m || !row_offsets) return DFQ_ERR_INVALID_ARG;
@@ -338,7 +415,7 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   dfq_serve_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else dfq_serve_config_default(&cfg);
-  if (!shape_ok(cfg.xcds_per_team, cfg.block_size) || cfg.reserved || cfg.ring_slots < 2 ||
+  if (!shape_ok(cfg.xcds_per_team, cfg.block_size) || cfg.groups > 1 || cfg.ring_slots < 2 ||
       cfg.ring_slots > 65536 || (cfg.ring_slots & (cfg.ring_slots - 1)))
     return DFQ_ERR_INVALID_ARG;
   std::vector<ServeGraph> graphs(num_graphs);
@@ -359,10 +436,13 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
     if (m[g] > kMaxEntries || !csr_valid(off, col, n[g], m[g])) return DFQ_ERR_RANGE;
     graphs[g].symmetric = is_symmetric(off, col, n[g]) ? 1u : 0u;
   }
-  uint64_t iso_words = 0;
+  uint64_t iso_words = 0, obase = 0;
   for (auto &gr : graphs) {
     gr.iso_base = gr.symmetric ? iso_words : ~0ull;
     if (gr.symmetric) iso_words += ((uint64_t)gr.n + 31) / 32;
+    gr.obase = obase;
+    obase += gr.n;
+    gr.nz = gr.n;
   }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DFQ_ERR_NO_DEVICE;
@@ -415,17 +495,49 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
     SFAIL(hipMalloc(&s->d_iso, sizeof(uint32_t) * iso.size()));
     SFAIL(hipMemcpy(s->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));
   }
+  if (cfg.groups) {
+    // group tasks search degree-ranked copies of the graphs
+    SFAIL(hipMalloc(&s->d_off_r, sizeof(int64_t) * vtot));
+    SFAIL(hipMalloc(&s->d_col_r, sizeof(uint32_t) * (etot + 4)));
+    SFAIL(hipMalloc(&s->d_o2r, sizeof(uint32_t) * (obase ? obase : 1)));
+    std::vector<int64_t> off_r;
+    std::vector<uint32_t> col_r, o2r, r2o;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+      ServeGraph &gr = graphs[g];
+      try {
+        off_r.resize((size_t)gr.n + 1);
+        col_r.resize(gr.m ? gr.m : 1);
+        o2r.resize(gr.n);
+        r2o.resize(gr.n);
+      } catch (...) {
+        return fail(DFQ_ERR_NOMEM);
+      }
+      rank_relabel(row_offsets + gr.vbase, col_idx + gr.ebase, gr.n, o2r.data(), r2o.data(), off_r.data(),
+                   col_r.data());
+      uint32_t nz = gr.n;
+      while (nz > 0 && off_r[nz - 1] == off_r[nz]) --nz;
+      gr.nz = nz;
+      SFAIL(hipMemcpy(s->d_off_r + gr.vbase, off_r.data(), sizeof(int64_t) * ((size_t)gr.n + 1),
+                      hipMemcpyHostToDevice));
+      if (gr.m) SFAIL(hipMemcpy(s->d_col_r + gr.ebase, col_r.data(), sizeof(uint32_t) * gr.m, hipMemcpyHostToDevice));
+      SFAIL(hipMemcpy(s->d_o2r + gr.obase, o2r.data(), sizeof(uint32_t) * gr.n, hipMemcpyHostToDevice));
+    }
+    s->graphs = graphs;
+  }
   SFAIL(hipMemcpy(s->d_graphs, graphs.data(), sizeof(ServeGraph) * num_graphs, hipMemcpyHostToDevice));
   SFAIL(hipMalloc(&s->d_ctl, sizeof(ServeCtl)));
   SFAIL(hipMemset(s->d_ctl, 0, sizeof(ServeCtl)));
-  // arena: by default four working sets of the largest graph per team
+  // arena: by default four working sets of the largest graph per team (two
+  // full groups per team in group mode)
   uint64_t arena = cfg.arena_bytes;
   if (arena == 0) {
     uint64_t fmax = 0;
     for (uint32_t g = 0; g < num_graphs; ++g)
-      fmax = std::max(fmax, footprint_of(graphs[g], s->plans[g], cfg.xcds_per_team, kServeParents, nullptr));
+      fmax = std::max(fmax, cfg.groups ? group_footprint(graphs[g], s->plans[g], cfg.xcds_per_team, kMsMaxSources)
+                                       : footprint_of(graphs[g], s->plans[g], cfg.xcds_per_team, kServeParents,
+                                                      nullptr));
     const uint64_t teams = cfg.xcds_per_team ? (8 + cfg.xcds_per_team - 1) / cfg.xcds_per_team : 1;
-    arena = 4 * teams * fmax;
+    arena = (cfg.groups ? 2 : 4) * teams * fmax;
     size_t free_b = 0, total_b = 0;
     SFAIL(hipMemGetInfo(&free_b, &total_b));
     arena = std::min<uint64_t>(arena, free_b / 2);
@@ -437,6 +549,8 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   // ring, results, stop word and status in one pinned, coherent block
   const uint32_t R = cfg.ring_slots;
   const size_t block_bytes = sizeof(ServeRequest) * R + sizeof(ServeResult) * R + 256 + sizeof(ServeStatus);
+  // group blocks: sources and per-search results of every ring slot
+  const size_t gblock_bytes = (sizeof(uint32_t) + sizeof(ServeGroupResult)) * kMsMaxSources * R;
   SFAIL(hipHostMalloc(&s->h_block, block_bytes, hipHostMallocMapped | hipHostMallocCoherent));
   memset(s->h_block, 0, block_bytes);
   uint8_t *hb = (uint8_t *)s->h_block;
@@ -449,6 +563,12 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   uint8_t *db = (uint8_t *)dblock;
   SFAIL(hipHostMalloc((void **)&s->h_stage, kStageBytes, hipHostMallocMapped | hipHostMallocCoherent));
   SFAIL(hipHostGetDevicePointer((void **)&s->d_stage, s->h_stage, 0));
+  void *gblock = nullptr, *d_gblock = nullptr;
+  SFAIL(hipHostMalloc(&gblock, gblock_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+  memset(gblock, 0, gblock_bytes);
+  s->h_gres = (ServeGroupResult *)gblock;
+  s->h_gsrc = (uint32_t *)((uint8_t *)gblock + sizeof(ServeGroupResult) * kMsMaxSources * R);
+  SFAIL(hipHostGetDevicePointer(&d_gblock, gblock, 0));
   s->ring.assign(R, RingSlot());
 
   ServeArgs &a = s->args;
@@ -465,6 +585,12 @@ int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
   a.ctl = s->d_ctl;
   a.arena = s->d_arena;
   a.ring = R;
+  a.off_r = s->d_off_r;
+  a.col_r = s->d_col_r;
+  a.o2r = s->d_o2r;
+  a.gres = (ServeGroupResult *)d_gblock;
+  a.gsrc = (const uint32_t *)((uint8_t *)d_gblock + sizeof(ServeGroupResult) * kMsMaxSources * R);
+  a.bu_div = kMsBuDiv;
 #undef SFAIL
   *out = s;
   return DFQ_OK;
@@ -523,7 +649,7 @@ int dfq_server_start(dfq_server *s) {
   if (s->running || s->wedged) return DFQ_ERR_STATE;
   SHIP(hipSetDevice(s->device));
   int per_cu = 0;
-  SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel_for(s->cfg.block_size),
+  SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, serve_kernel_for(s->cfg.block_size, s->cfg.groups != 0),
                                                     (int)s->cfg.block_size, 0));
   hipDeviceProp_t prop;
   SHIP(hipGetDeviceProperties(&prop, s->device));
@@ -545,7 +671,7 @@ int dfq_server_start(dfq_server *s) {
   SHIP(hipMemsetAsync(&s->d_ctl->head, 0, sizeof(BarrierWord), s->stream));
   s->args.xpt = s->cfg.xcds_per_team;
   void *kargs[] = {&s->args};
-  SHIP(hipLaunchCooperativeKernel(serve_kernel_for(s->cfg.block_size), dim3(s->grid), dim3(s->cfg.block_size),
+  SHIP(hipLaunchCooperativeKernel(serve_kernel_for(s->cfg.block_size, s->cfg.groups != 0), dim3(s->grid), dim3(s->cfg.block_size),
                                   kargs, 0, s->stream));
   s->running = true;
   // the teams report in once the launch is resident
@@ -599,13 +725,11 @@ int dfq_server_stop(dfq_server *s) {
   for (uint32_t i = 0; i < s->cfg.ring_slots; ++i) {
     RingSlot &rs = s->ring[i];
     if (!rs.busy) continue;
-    if (!rs.copy) {
-      auto it = s->queries.find(rs.ticket);
-      if (it != s->queries.end()) {
-        s->arena.release(it->second.work, it->second.bytes);
-        s->queries.erase(it);
+    if (!rs.copy)
+      for (uint32_t i = 0; i < rs.count; ++i) {
+        auto it = s->queries.find(rs.qid0 + i);
+        if (it != s->queries.end()) release_query(s, it);
       }
-    }
     rs.busy = false;
     rs.copy = false;
     if (rc == DFQ_OK) rc = DFQ_ERR_STATE;
@@ -613,8 +737,12 @@ int dfq_server_stop(dfq_server *s) {
   return rc;
 }
 
+int dfq_server_submit_group(dfq_server *s, uint32_t graph, uint32_t count, const uint32_t *sources, uint32_t flags,
+                            uint64_t *tickets);
+
 int dfq_server_submit(dfq_server *s, uint32_t graph, uint32_t source, uint32_t flags, uint64_t *ticket) {
   if (!s || !ticket || graph >= s->graphs.size() || (flags & ~kServeParents)) return DFQ_ERR_INVALID_ARG;
+  if (s->cfg.groups) return dfq_server_submit_group(s, graph, 1, &source, flags, ticket);
   if (source >= s->graphs[graph].n) return DFQ_ERR_RANGE;
   if (!s->running) return DFQ_ERR_STATE;
   const ServeGraph &g = s->graphs[graph];
@@ -644,10 +772,69 @@ int dfq_server_submit(dfq_server *s, uint32_t graph, uint32_t source, uint32_t f
   q.source = source;
   q.flags = flags;
   q.item_cap = item_cap;
-  const uint64_t T = s->next_ticket;
-  s->queries[T] = q;
-  publish(s, req, false);
-  *ticket = T;
+  q.ring_ticket = s->next_ticket;
+  const uint64_t qid = s->next_qid++;
+  s->queries[qid] = q;
+  publish(s, req, false, qid, 1);
+  *ticket = qid;
+  return DFQ_OK;
+}
+
+int dfq_server_submit_group(dfq_server *s, uint32_t graph, uint32_t count, const uint32_t *sources, uint32_t flags,
+                            uint64_t *tickets) {
+  if (!s || !tickets || !sources || graph >= s->graphs.size() || flags) return DFQ_ERR_INVALID_ARG;
+  if (!s->cfg.groups || count == 0 || count > kMsMaxSources) return DFQ_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < count; ++i)
+    if (sources[i] >= s->graphs[graph].n) return DFQ_ERR_RANGE;
+  if (!s->running) return DFQ_ERR_STATE;
+  const ServeGraph &g = s->graphs[graph];
+  const dfq_query_plan &p = s->plans[graph];
+  const uint32_t item_cap = group_item_cap(g, p);
+  const uint64_t bytes = group_footprint(g, p, s->cfg.xcds_per_team, count);
+  if (bytes > s->arena.size()) return DFQ_ERR_CAPACITY;
+  const uint32_t idx = (uint32_t)(s->next_ticket % s->cfg.ring_slots);
+  if (!collect(s, idx)) return DFQ_ERR_BUSY;
+  uint64_t work = 0;
+  if (!s->arena.alloc(bytes, &work)) return DFQ_ERR_BUSY;
+  ServeRequest req;
+  memset(&req, 0, sizeof(req));
+  req.work = work;
+  req.graph = graph;
+  req.source = count;
+  req.flags = kServeGroup | (p.direction_opt ? kServeDirOpt : 0u);
+  req.item_cap = item_cap;
+  req.small_max = p.small_max;
+  req.medium_max = p.medium_max;
+  req.large_max = p.large_max;
+  req.alpha = p.alpha;
+  req.beta = p.beta;
+  // the sources travel in the slot's host-mapped group block
+  memcpy(s->h_gsrc + (size_t)idx * kMsMaxSources, sources, sizeof(uint32_t) * count);
+  const uint64_t gid = s->next_group++;
+  GroupRange gr;
+  gr.work = work;
+  gr.bytes = bytes;
+  gr.held = count;
+  s->groups_held[gid] = gr;
+  const uint64_t qid0 = s->next_qid;
+  s->next_qid += count;
+  for (uint32_t i = 0; i < count; ++i) {
+    Query q;
+    q.work = work;
+    q.bytes = bytes;
+    q.group = gid;
+    q.member = i;
+    q.graph = graph;
+    q.source = sources[i];
+    q.flags = 0;
+    q.item_cap = item_cap;
+    q.count = count;
+    q.ring_ticket = s->next_ticket;
+    s->queries[qid0 + i] = q;
+    tickets[i] = qid0 + i;
+  }
+  // the host's stores to the group block precede the publication's release
+  publish(s, req, false, qid0, count);
   return DFQ_OK;
 }
 
@@ -657,7 +844,7 @@ int dfq_server_poll(dfq_server *s, uint64_t ticket, dfq_serve_result *result) {
   if (!q) return DFQ_ERR_INVALID_ARG;
   if (!q->done) return DFQ_ERR_PENDING;
   if (result) *result = q->result;
-  return DFQ_OK;
+  return q->status;
 }
 
 int dfq_server_wait(dfq_server *s, uint64_t ticket, uint32_t timeout_ms, dfq_serve_result *result) {
@@ -677,7 +864,13 @@ int dfq_server_levels(dfq_server *s, uint64_t ticket, uint32_t *level_out, uint3
   Query *q = find(s, ticket);
   if (!q) return DFQ_ERR_INVALID_ARG;
   if (!q->done) return DFQ_ERR_PENDING;
+  if (q->status != DFQ_OK) return q->status;
   const ServeGraph &g = s->graphs[q->graph];
+  if (q->group) {
+    if (parent_out) return DFQ_ERR_INVALID_ARG;
+    const ServeGroupLayout l = serve_group_layout(g.n, q->count, q->item_cap, serve_segments(s->cfg.xcds_per_team));
+    return copy_out(s, q->work + l.level + 4ull * l.n_pad * q->member, level_out, 4ull * g.n);
+  }
   const bool parents = (q->flags & kServeParents) != 0;
   if (parent_out && !parents) return DFQ_ERR_INVALID_ARG;
   // the layout the query was admitted with (the shape cannot change while
@@ -694,10 +887,121 @@ int dfq_server_release(dfq_server *s, uint64_t ticket) {
   Query *q = find(s, ticket);
   if (!q) return DFQ_ERR_INVALID_ARG;
   if (!q->done) return DFQ_ERR_PENDING;
-  s->arena.release(q->work, q->bytes);
-  s->queries.erase(ticket);
+  release_query(s, s->queries.find(ticket));
   return DFQ_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// The serving loop of a group-mode server: arrivals wait per graph; while
+// fewer than `teams + 2` group tasks are in flight, the graph of the oldest
+// waiting query sends up to 64 of its waiting queries (oldest first) as one
+// group task.  Queries that arrive while the teams are busy so wait for the
+// next free team and share its traversal.
+int replay_groups(dfq_server *s, uint32_t nq, const uint32_t *graph, const uint32_t *source, const double *arrival_us,
+                  uint32_t *const *level_out, dfq_serve_record *records) {
+  const uint32_t ngraphs = (uint32_t)s->graphs.size();
+  std::vector<std::vector<uint32_t>> waiting(ngraphs);  // per graph, arrival order
+  std::vector<size_t> head(ngraphs, 0);                 // first waiting entry not yet sent
+  struct Live {
+    std::vector<uint32_t> q;
+    std::vector<uint64_t> t;
+  };
+  std::vector<Live> live;
+  const uint32_t teams = std::max<uint32_t>(1u, __atomic_load_n(&s->h_status->teams, __ATOMIC_ACQUIRE));
+  const size_t max_live = teams + 2;
+  const auto t0 = Clock::now();
+  uint32_t arrived = 0, oldest = 0, finished = 0;
+  std::vector<uint8_t> sent(nq, 0);
+  std::vector<uint32_t> members;
+  std::vector<uint32_t> srcs;
+  std::vector<uint64_t> tks(kMsMaxSources);
+  uint64_t idle = 0;
+  while (finished < nq) {
+    bool progress = false;
+    const double now = us_since(t0);
+    while (arrived < nq && arrival_us[arrived] <= now) {
+      waiting[graph[arrived]].push_back(arrived);
+      ++arrived;
+    }
+    while (live.size() < max_live) {
+      while (oldest < arrived && sent[oldest]) ++oldest;
+      if (oldest >= arrived) break;
+      const uint32_t g = graph[oldest];
+      // the largest group whose working set the arena can hold
+      uint32_t cap = kMsMaxSources;
+      while (cap > 1 && group_footprint(s->graphs[g], s->plans[g], s->cfg.xcds_per_team, cap) > s->arena.size())
+        cap /= 2;
+      members.clear();
+      srcs.clear();
+      for (size_t k = head[g]; k < waiting[g].size() && members.size() < cap; ++k) {
+        members.push_back(waiting[g][k]);
+        srcs.push_back(source[waiting[g][k]]);
+      }
+      const int rc = dfq_server_submit_group(s, g, (uint32_t)members.size(), srcs.data(), 0, tks.data());
+      if (rc == DFQ_ERR_BUSY && live.empty() && !any_running(s)) return DFQ_ERR_BUSY;
+      if (rc == DFQ_ERR_BUSY) break;
+      if (rc != DFQ_OK) return rc;
+      const double admit = us_since(t0);
+      Live l;
+      for (size_t k = 0; k < members.size(); ++k) {
+        const uint32_t q = members[k];
+        sent[q] = 1;
+        records[q].arrival_us = arrival_us[q];
+        records[q].admit_us = admit;
+        l.q.push_back(q);
+        l.t.push_back(tks[k]);
+      }
+      head[g] += members.size();
+      live.push_back(std::move(l));
+      progress = true;
+    }
+    for (size_t i = 0; i < live.size();) {
+      dfq_serve_result r;
+      const int rc = dfq_server_poll(s, live[i].t[0], &r);
+      if (rc == DFQ_ERR_PENDING) {
+        ++i;
+        continue;
+      }
+      const double done = us_since(t0);
+      for (size_t k = 0; k < live[i].q.size(); ++k) {
+        const uint32_t q = live[i].q[k];
+        const uint64_t tk = live[i].t[k];
+        const int prc = dfq_server_poll(s, tk, &r);
+        if (prc != DFQ_OK) return prc;
+        dfq_serve_record &rec = records[q];
+        rec.done_us = done;
+        rec.service_us = r.service_us;
+        rec.reached = r.reached;
+        rec.edges = r.edges;
+        rec.depth = r.depth;
+        rec.team = r.team;
+        if (level_out && level_out[q]) {
+          const int crc = dfq_server_levels(s, tk, level_out[q], nullptr);
+          if (crc != DFQ_OK) return crc;
+        }
+        const int rrc = dfq_server_release(s, tk);
+        if (rrc != DFQ_OK) return rrc;
+        ++finished;
+      }
+      live[i] = std::move(live.back());
+      live.pop_back();
+      progress = true;
+    }
+    if (progress) {
+      idle = 0;
+    } else if ((++idle & 4095) == 0) {
+      if (kernel_ended(s)) return DFQ_ERR_STATE;
+    }
+  }
+  return DFQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int dfq_server_replay(dfq_server *s, uint32_t nq, const uint32_t *graph, const uint32_t *source,
                       const double *arrival_us, uint32_t flags, uint32_t *const *level_out,
@@ -708,11 +1012,17 @@ int dfq_server_replay(dfq_server *s, uint32_t nq, const uint32_t *graph, const u
     if (graph[q] >= s->graphs.size()) return DFQ_ERR_INVALID_ARG;
     if (source[q] >= s->graphs[graph[q]].n) return DFQ_ERR_RANGE;
     if (q && arrival_us[q] < arrival_us[q - 1]) return DFQ_ERR_INVALID_ARG;
-    const uint64_t fp = footprint_of(s->graphs[graph[q]], s->plans[graph[q]], s->cfg.xcds_per_team, flags, nullptr);
+    const uint64_t fp = s->cfg.groups
+                            ? group_footprint(s->graphs[graph[q]], s->plans[graph[q]], s->cfg.xcds_per_team, 1)
+                            : footprint_of(s->graphs[graph[q]], s->plans[graph[q]], s->cfg.xcds_per_team, flags, nullptr);
     if (fp > s->arena.size()) return DFQ_ERR_CAPACITY;
   }
   if (!s->running) return DFQ_ERR_STATE;
   memset(records, 0, sizeof(dfq_serve_record) * nq);
+  if (s->cfg.groups) {
+    if (flags) return DFQ_ERR_INVALID_ARG;  // group tasks write no parents
+    return replay_groups(s, nq, graph, source, arrival_us, level_out, records);
+  }
   std::vector<std::pair<uint32_t, uint64_t>> live;
   live.reserve(s->cfg.ring_slots);
   const auto t0 = Clock::now();

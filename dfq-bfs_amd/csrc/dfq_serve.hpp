@@ -12,12 +12,14 @@
 #pragma once
 
 #include "dfq_kernels.hpp"
+#include "dfq_msbfs.hpp"
 
 namespace dfq {
 
 constexpr uint32_t kServeParents = 1u;   // public flag DFQ_SERVE_PARENTS
 constexpr uint32_t kServeDirOpt = 2u;    // internal: direction optimisation allowed
 constexpr uint32_t kServeCopy = 0x100u;  // internal: a copy task, not a search
+constexpr uint32_t kServeGroup = 0x200u; // internal: up to 64 searches of one graph in one traversal
 constexpr int kMaxTeams = kMaxXcd;
 
 struct ServeGraph {
@@ -27,6 +29,19 @@ struct ServeGraph {
   uint32_t n;
   uint32_t symmetric;
   uint64_t iso_base;   // first word of its isolated-vertex bitmap, ~0 if none
+  uint64_t obase;      // first entry of its vertex -> rank map (degree-ranked copy)
+  uint32_t nz;         // vertices with entries (ranks [0, nz))
+  uint32_t pad;
+};
+
+// A group task (kServeGroup) holds `source` searches (1..64) of `graph`,
+// their sources in the host-mapped group block of the ring slot; each
+// search's reached count, entries and depth go back there too.
+struct ServeGroupResult {
+  uint64_t reached;
+  uint64_t edges;
+  uint32_t depth;
+  uint32_t pad;
 };
 
 // Written by the host, read by the team leader.  seq = ticket + 1 publishes.
@@ -76,6 +91,12 @@ struct ServeLayout {
   uint64_t level, parent, vis, snap, fb, qseg0, qseg1, items, bytes;
 };
 
+// Offsets of a group task's arrays inside its arena range: `count` level
+// arrays of n_pad entries, then the multi-source search's working set.
+struct ServeGroupLayout {
+  uint64_t level, n_pad, seen, f, planes, qb, items, counters, depth, sources, bytes;
+};
+
 __host__ __device__ inline uint64_t serve_align(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
 
 // Work items of one level fit in item_cap per segment: a row of d > large_max
@@ -110,6 +131,24 @@ __host__ __device__ inline ServeLayout serve_layout(uint32_t n, uint32_t item_ca
   return l;
 }
 
+__host__ __device__ inline ServeGroupLayout serve_group_layout(uint32_t n, uint32_t count, uint32_t item_cap,
+                                                              uint32_t segs) {
+  ServeGroupLayout l;
+  l.n_pad = ((uint64_t)n + 63) & ~(uint64_t)63;
+  uint64_t o = 0;
+  l.level = o;    o += serve_align(4ull * l.n_pad * count);
+  l.seen = o;     o += serve_align(8ull * n);
+  l.f = o;        o += serve_align(16ull * n);
+  l.planes = o;   o += serve_align(8ull * kMsPlanes * n);
+  l.qb = o;       o += serve_align(4ull * 3 * segs * n);
+  l.items = o;    o += serve_align(8ull * segs * item_cap);
+  l.counters = o; o += serve_align(8ull * 2 * kMsMaxSources);
+  l.depth = o;    o += serve_align(4ull * kMsMaxSources);
+  l.sources = o;  o += serve_align(4ull * kMsMaxSources);
+  l.bytes = o;
+  return l;
+}
+
 struct ServeArgs {
   const int64_t *off;           // concatenated row offsets
   const uint32_t *col;          // concatenated columns
@@ -125,6 +164,13 @@ struct ServeArgs {
   uint8_t *arena;
   uint32_t ring;                // power of two
   uint32_t xpt;                 // XCDs per team (0 = all)
+  const int64_t *off_r;         // degree-ranked copies of the graphs (group tasks), indexed like off
+  const uint32_t *col_r;        // and like col
+  const uint32_t *o2r;          // vertex -> rank, graph g's from graphs[g].obase
+  const uint32_t *gsrc;         // host-mapped [ring][64] sources of group tasks
+  ServeGroupResult *gres;       // host-mapped [ring][64] results of group tasks
+  uint32_t bu_div;              // group tasks: bottom-up when frontier out-edges > m / bu_div
+  uint32_t pad;
 };
 
 __device__ __forceinline__ uint64_t ld_system(const uint64_t *p) {
@@ -134,7 +180,82 @@ __device__ __forceinline__ uint32_t ld_system(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// A group task on the team: the multi-source search of up to 64 queries of
+// one graph, then their results into the host-mapped group block and the
+// ring.
 template <int BLOCK>
+__device__ __forceinline__ void serve_group_task(const ServeArgs &s, const ServeRequest &d, SharedAll<BLOCK> &sh,
+                                              Team &team, uint32_t id, uint64_t t0) {
+  ServeCtl *c = s.ctl;
+  GridBarrier *bar = &c->team[id].bar;
+  const uint32_t tid = threadIdx.x;
+    const ServeGraph g = s.graphs[d.graph];
+    const uint32_t segs = serve_segments(s.xpt);
+    const ServeGroupLayout gl = serve_group_layout(g.n, d.source, d.item_cap, segs);
+    uint8_t *w = s.arena + d.work;
+    MsArgs ms;
+    ms.off = s.off_r + g.vbase;
+    ms.col = s.col_r + g.ebase;
+    ms.col_bu = nullptr;
+    ms.o2r = s.o2r + g.obase;
+    ms.nz = g.nz;
+    ms.n = g.n;
+    ms.count = d.source;
+    ms.m = g.m;
+    ms.sources = (const uint32_t *)(w + gl.sources);
+    ms.level = (uint32_t *)(w + gl.level);
+    ms.level_stride = gl.n_pad;
+    ms.seen = (uint64_t *)(w + gl.seen);
+    ms.f = (uint64_t *)(w + gl.f);
+    ms.planes = (uint64_t *)(w + gl.planes);
+    ms.qb = (uint32_t *)(w + gl.qb);
+    ms.segs = segs;
+    ms.items = (uint2 *)(w + gl.items);
+    ms.item_cap = d.item_cap;
+    ms.ctl = &c->team[id];
+    ms.counters = (uint64_t *)(w + gl.counters);
+    ms.depth = (uint32_t *)(w + gl.depth);
+    ms.small_max = d.small_max;
+    ms.medium_max = d.medium_max;
+    ms.large_max = d.large_max;
+    ms.bottom_up = ((d.flags & kServeDirOpt) && g.symmetric) ? 1u : 0u;
+    ms.bu_div = s.bu_div;
+    ms.prof_levels = 0;
+    ms.prof = nullptr;
+    ms_grid_search<BLOCK>(ms, sh, team);
+    // outputs reach memory the host's copies read before the result appears
+    grid_barrier<true>(bar, team.bst, 0, sh.bcast);
+    if (sh.team_rank == 0 && tid == 0) {
+      const uint64_t tk = c->ticket[id];
+      const uint32_t slot = (uint32_t)(tk & (s.ring - 1));
+      ServeGroupResult *gr = s.gres + (size_t)slot * kMsMaxSources;
+      uint64_t reached = 0, edges = 0;
+      for (uint32_t i = 0; i < d.source; ++i) {
+        const uint64_t ri = ms.counters[i], ei = ms.counters[kMsMaxSources + i];
+        __hip_atomic_store(&gr[i].reached, ri, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&gr[i].edges, ei, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&gr[i].depth, ms.depth[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        reached += ri;
+        edges += ei;
+      }
+      ServeResult *o = &s.res[slot];
+      o->reached = reached;
+      o->depth = ms.ctl->depth;
+      o->bu_levels = ms.ctl->bu_levels;
+      o->edges = edges;
+      o->team = id;
+      o->err = ms.ctl->err;
+      o->t_start_ns = t0;
+      o->t_end_ns = now_ns();
+      __hip_atomic_store(&o->done, tk + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// kGroups: the launch serves group tasks (multi-source searches) and copy
+// tasks; otherwise single searches and copy tasks.  A launch carries only
+// the search path it runs (both inlined together would not fit the
+// registers of 1024-thread workgroups).
+template <int BLOCK, bool kGroups>
 __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
@@ -177,6 +298,15 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
         ServeRequest d = *r;
         c->desc[id] = d;
         c->ticket[id] = t;
+        if (d.flags & kServeGroup) {
+          // the group's sources move from the host-mapped slot into the
+          // working set (the team barrier below publishes them)
+          const ServeGraph gg = s.graphs[d.graph];
+          const ServeGroupLayout gl = serve_group_layout(gg.n, d.source, d.item_cap, serve_segments(s.xpt));
+          uint32_t *dst = (uint32_t *)(s.arena + d.work + gl.sources);
+          const uint32_t *src = s.gsrc + (size_t)(t & (s.ring - 1)) * kMsMaxSources;
+          for (uint32_t i = 0; i < d.source; ++i) dst[i] = ld_system(&src[i]);
+        }
       }
     }
     if (grid_barrier(bar, team.bst, stop, sh.bcast)) break;
@@ -201,6 +331,10 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
         o->t_end_ns = now_ns();
         __hip_atomic_store(&o->done, tk + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       }
+      continue;
+    }
+    if constexpr (kGroups) {
+      serve_group_task<BLOCK>(s, d, sh, team, id, t0);
       continue;
     }
     const ServeGraph g = s.graphs[d.graph];

@@ -156,6 +156,43 @@ inline void order_rows_by_degree(const int64_t *off, const uint32_t *col, uint32
   });
 }
 
+// Degree-ranked copy of a CSR for multi-source searches: vertex v gets rank
+// o2r[v] (decreasing degree, ties by id; r2o is the inverse), row r of
+// (off_r, col_r) is row r2o[r] with its neighbours given as ranks, sorted
+// ascending, i.e. highest degree first.  The frontier masks of the
+// highest-degree vertices - the tails of most edges in a skewed graph - then
+// sit together at the front of every per-vertex array, where they stay in
+// cache; and a row read in order meets its hubs first.
+inline void rank_relabel(const int64_t *off, const uint32_t *col, uint32_t n, uint32_t *o2r, uint32_t *r2o,
+                         int64_t *off_r, uint32_t *col_r) {
+  std::vector<uint64_t> key(n);
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t v = b; v < e; ++v) {
+      const uint64_t d = (uint64_t)std::min<int64_t>(off[v + 1] - off[v], 0xffffffffll);
+      key[v] = ((0xffffffffull - d) << 32) | v;
+    }
+  });
+  std::sort(key.begin(), key.end());
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t r = b; r < e; ++r) {
+      const uint32_t v = (uint32_t)key[r];
+      r2o[r] = v;
+      o2r[v] = (uint32_t)r;
+    }
+  });
+  off_r[0] = 0;
+  for (uint32_t r = 0; r < n; ++r) off_r[r + 1] = off_r[r] + (off[r2o[r] + 1] - off[r2o[r]]);
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t r = b; r < e; ++r) {
+      const uint32_t v = r2o[r];
+      uint32_t *dst = col_r + off_r[r];
+      const int64_t len = off[v + 1] - off[v];
+      for (int64_t k = 0; k < len; ++k) dst[k] = o2r[col[off[v] + k]];
+      std::sort(dst, dst + len);
+    }
+  });
+}
+
 // First four entries of every row of `ordered` (rows by neighbour degree),
 // four words per vertex; words past a row's end are 0 (never tested).
 inline void row_heads(const int64_t *off, const uint32_t *ordered, uint32_t n, uint32_t *out) {

@@ -98,7 +98,7 @@ class Config(C.Structure):
                 ("small_max", C.c_uint32), ("medium_max", C.c_uint32),
                 ("large_max", C.c_uint32), ("direction_opt", C.c_uint32),
                 ("bu_enter_div", C.c_uint32), ("bu_exit_div", C.c_uint32),
-                ("launch_mode", C.c_uint32)]
+                ("launch_mode", C.c_uint32), ("multi_bu_div", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -172,6 +172,11 @@ _sig("dfq_bfs_device_seq", C.c_int, _P, C.c_uint32, _u32p, _P, C.c_uint64, _P, C
      C.POINTER(Stats), C.POINTER(C.c_double))
 _sig("dfq_bfs_profile", C.c_int, _P, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32,
      C.POINTER(Stats))
+_sig("dfq_bfs_device_multi", C.c_int, _P, C.c_uint32, _u32p, _P, C.c_uint64, C.POINTER(Stats),
+     C.POINTER(C.c_double))
+_sig("dfq_bfs_multi_profile", C.c_int, _P, C.c_uint32, _u32p, _P, C.c_uint64, C.POINTER(C.c_uint64),
+     C.c_uint32, C.POINTER(Stats))
+MULTI_MAX_SOURCES = 64
 _sig("dfq_graph_is_symmetric", C.c_int, _P)
 _sig("dfq_device_alloc", C.c_int, C.c_size_t, C.POINTER(_P))
 _sig("dfq_device_free", C.c_int, _P)
@@ -433,6 +438,32 @@ class DeviceGraph:
                                      parent_stride, st, C.byref(ms)), "dfq_bfs_device_seq")
         return list(st[:so.size]), ms.value
 
+    def bfs_device_multi(self, sources, d_level: int, level_stride: int | None = None):
+        """Up to 64 searches in one traversal (search i writes d_level + i *
+        level_stride, default n); returns (per-search Stats, launch ms)."""
+        so = np.ascontiguousarray(sources, np.uint32)
+        st = (Stats * max(so.size, 1))()
+        ms = C.c_double()
+        check(lib.dfq_bfs_device_multi(self._h, so.size, _u32(so), d_level,
+                                       self.n if level_stride is None else level_stride, st,
+                                       C.byref(ms)), "dfq_bfs_device_multi")
+        return list(st[:so.size]), ms.value
+
+    MULTI_PROFILE_FIELDS = ("frontier", "edges", "bottom_up", "active", "items", "commit_ns",
+                            "expand_ns", "next")
+
+    def multi_profile(self, sources, d_level: int, max_levels: int = 64):
+        """Per-level counters of one multi-source search (list of dicts)."""
+        so = np.ascontiguousarray(sources, np.uint32)
+        rows = np.zeros((max_levels, len(self.MULTI_PROFILE_FIELDS)), np.uint64)
+        st = (Stats * max(so.size, 1))()
+        check(lib.dfq_bfs_multi_profile(self._h, so.size, _u32(so), d_level, self.n,
+                                        rows.ctypes.data_as(C.POINTER(C.c_uint64)), max_levels, st),
+              "dfq_bfs_multi_profile")
+        depth = max(s.depth for s in st[:so.size])
+        return ([dict(zip(self.MULTI_PROFILE_FIELDS, map(int, rows[i]))) for i in range(min(depth, max_levels))],
+                list(st[:so.size]))
+
     def bfs_device(self, source: int, d_level: int, d_parent: int | None = None) -> Stats:
         """BFS into caller-owned device buffers (raw device addresses)."""
         st = Stats()
@@ -630,7 +661,7 @@ SERVE_PARENTS = 0x1
 
 class ServeConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("xcds_per_team", C.c_uint32), ("ring_slots", C.c_uint32),
-                ("reserved", C.c_uint32), ("arena_bytes", C.c_uint64)]
+                ("groups", C.c_uint32), ("arena_bytes", C.c_uint64)]
 
 
 class ServerInfo(C.Structure):
@@ -659,6 +690,10 @@ class ServeRecord(C.Structure):
 
 _sig("dfq_serve_config_default", C.c_int, C.POINTER(ServeConfig))
 _sig("dfq_serve_footprint", C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32)
+_sig("dfq_serve_group_footprint", C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32)
+_sig("dfq_server_submit_group", C.c_int, _P, C.c_uint32, C.c_uint32, _u32p, C.c_uint32,
+     C.POINTER(C.c_uint64))
+SERVE_GROUP_MAX = 64
 _sig("dfq_server_create", C.c_int, C.c_uint32, _u32p, C.POINTER(C.c_uint64), _i64p, _u32p,
      C.POINTER(ServeConfig), C.POINTER(_P))
 _sig("dfq_server_destroy", None, _P)
@@ -691,9 +726,10 @@ class Server:
     """Resident graphs served by one persistent launch (continuous batching)."""
 
     def __init__(self, graphs, *, xcds_per_team: int | None = None, block_size: int | None = None,
-                 ring_slots: int | None = None, arena_bytes: int | None = None):
+                 ring_slots: int | None = None, arena_bytes: int | None = None, groups: bool = False):
         self.graphs = list(graphs)
         cfg = default_serve_config()
+        cfg.groups = 1 if groups else 0
         if xcds_per_team is not None:
             cfg.xcds_per_team = xcds_per_team
         if block_size is not None:
@@ -758,6 +794,14 @@ class Server:
         t = C.c_uint64()
         check(lib.dfq_server_submit(self._h, graph, source, flags, C.byref(t)), "dfq_server_submit")
         return t.value
+
+    def submit_group(self, graph: int, sources) -> list:
+        """Group-mode servers: up to 64 queries of one graph as one group task;
+        their tickets (DfqError with code ERR_BUSY when refused for now)."""
+        so = np.ascontiguousarray(sources, np.uint32)
+        t = (C.c_uint64 * max(so.size, 1))()
+        check(lib.dfq_server_submit_group(self._h, graph, so.size, _u32(so), 0, t), "dfq_server_submit_group")
+        return list(t[:so.size])
 
     def poll(self, ticket: int):
         r = ServeResult()

@@ -65,6 +65,9 @@ typedef struct dfq_config {
   uint32_t bu_enter_div;   /* alpha, > 0 when direction_opt                */
   uint32_t bu_exit_div;    /* beta, > 0 when direction_opt                 */
   uint32_t launch_mode;    /* DFQ_LAUNCH_*, default DFQ_LAUNCH_PLAIN        */
+  uint32_t multi_bu_div;   /* dfq_bfs_device_multi: a level expands bottom-up
+                              when its frontier's out-edges exceed m /
+                              multi_bu_div (0 = default, 2)                  */
 } dfq_config;
 
 /* How dfq_bfs / dfq_bfs_device / dfq_bfs_device_many / dfq_bfs_profile
@@ -188,6 +191,33 @@ int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources,
                        uint32_t *d_level, uint64_t level_stride,
                        uint32_t *d_parent, uint64_t parent_stride,
                        dfq_stats *stats, double *launch_ms);
+
+/* Up to DFQ_MULTI_MAX_SOURCES searches of one graph in one traversal
+ * (one cooperative launch): search i starts from sources[i] and writes its
+ * levels to d_level + i * level_stride (level_stride >= n).  Each vertex
+ * carries one bit per search (bit-parallel multi-source BFS), so a pass over
+ * an edge serves every search whose frontier holds its tail.  Levels are
+ * those of dfq_bfs for every search (sources may repeat).  No parents.
+ * stats[i]: depth and reached of search i; kernel_ms the launch's event
+ * time (also in *launch_ms when given), bottom_up_levels the traversal's.
+ * Levels deeper than 255 are not covered: DFQ_ERR_CAPACITY (run such
+ * sources with dfq_bfs_device). */
+#define DFQ_MULTI_MAX_SOURCES 64u
+int dfq_bfs_device_multi(dfq_graph *g, uint32_t count, const uint32_t *sources,
+                         uint32_t *d_level, uint64_t level_stride, dfq_stats *stats,
+                         double *launch_ms);
+
+/* The same with per-level counters, rows[max_levels][DFQ_MULTI_PROFILE_FIELDS]:
+ *   0 distinct vertices in the level's frontier (over all searches),
+ *   1 their out-degree sum, 2 1 if the level expanded bottom-up,
+ *   3 searches with a frontier at this level, 4 work items (rows cut into
+ *   256-entry pieces), 5 ns of the commit phase, 6 ns of the expansion
+ *   (both barriers included, timed by the first workgroup), 7 vertices put
+ *   in the next level's frontier.  Levels past max_levels are not recorded. */
+#define DFQ_MULTI_PROFILE_FIELDS 8
+int dfq_bfs_multi_profile(dfq_graph *g, uint32_t count, const uint32_t *sources,
+                          uint32_t *d_level, uint64_t level_stride, uint64_t *rows,
+                          uint32_t max_levels, dfq_stats *stats);
 
 /* BFS with per-level counters (the reference's frontier profiling build,
  * design/shared-q/atomic-sa/our-sep/upper-bound/bfs_ftr_prof.cu:98,134-140,

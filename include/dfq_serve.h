@@ -39,10 +39,17 @@ typedef struct dfq_serve_config {
   uint32_t block_size;      /* 512 or 1024 threads per workgroup                 */
   uint32_t xcds_per_team;   /* XCDs per team: 1, 2, 4 or 8; 0 = one team of all  */
   uint32_t ring_slots;      /* requests published at once (power of two, 2..65536) */
-  uint32_t reserved;        /* must be zero                                      */
+  uint32_t groups;          /* 0: each query is one search on a team; 1: queries
+                               of one graph share traversals in group tasks of up
+                               to DFQ_SERVE_GROUP_MAX (bit-parallel multi-source
+                               search, as dfq_bfs_device_multi); no parents, and
+                               levels deeper than 255 are refused               */
   uint64_t arena_bytes;     /* device memory for in-flight working sets; 0 = room
-                               for four queries on the largest graph per team     */
+                               for four queries (two full groups) on the largest
+                               graph per team                                     */
 } dfq_serve_config;
+
+#define DFQ_SERVE_GROUP_MAX 64u
 
 typedef struct dfq_server_info {
   uint32_t running;         /* 1 between start and stop                          */
@@ -95,6 +102,11 @@ int dfq_serve_config_default(dfq_serve_config *cfg);
 uint64_t dfq_serve_footprint(uint32_t n, uint64_t m, uint32_t large_max, uint32_t xcds_per_team,
                              uint32_t flags);
 
+/* Device bytes of one group task of `count` searches (1..DFQ_SERVE_GROUP_MAX)
+ * on a graph of n vertices and m CSR entries with the default plan: the
+ * count level arrays and the multi-source search's working set. */
+uint64_t dfq_serve_group_footprint(uint32_t n, uint64_t m, uint32_t count, uint32_t xcds_per_team);
+
 /* Uploads the graphs (checked as in dfq_graph_create) and allocates the
  * arena and the request ring.  cfg may be NULL (defaults). */
 int dfq_server_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m,
@@ -122,7 +134,17 @@ int dfq_server_stop(dfq_server *s);
  * footprint exceeds the whole arena. */
 int dfq_server_submit(dfq_server *s, uint32_t graph, uint32_t source, uint32_t flags,
                       uint64_t *ticket);
-/* DFQ_OK with the result when finished, DFQ_ERR_PENDING otherwise. */
+/* Group-mode servers: admits `count` queries (1..DFQ_SERVE_GROUP_MAX) of one
+ * graph as one group task that searches them in one traversal; tickets[i]
+ * is query i's ticket (poll, wait, levels and release work per ticket; the
+ * group's arena range is freed with its last member).  flags must be 0.
+ * DFQ_ERR_BUSY / DFQ_ERR_CAPACITY as dfq_server_submit.  In group mode
+ * dfq_server_submit admits a group of one. */
+int dfq_server_submit_group(dfq_server *s, uint32_t graph, uint32_t count, const uint32_t *sources,
+                            uint32_t flags, uint64_t *tickets);
+/* DFQ_OK with the result when finished, DFQ_ERR_PENDING otherwise;
+ * DFQ_ERR_CAPACITY (result still filled) for a group query whose traversal
+ * went past 255 levels (its levels are not available). */
 int dfq_server_poll(dfq_server *s, uint64_t ticket, dfq_serve_result *result);
 /* Polls until finished or timeout_ms passes (DFQ_ERR_TIMEOUT). */
 int dfq_server_wait(dfq_server *s, uint64_t ticket, uint32_t timeout_ms, dfq_serve_result *result);
@@ -136,6 +158,11 @@ int dfq_server_release(dfq_server *s, uint64_t ticket);
  * (non-decreasing); arrivals are admitted in order as slots and arena room
  * free up; finished queries are released at once (after their levels are
  * copied to level_out[q] when level_out and level_out[q] are not NULL).
+ * Group mode: arrivals wait per graph, and while fewer than teams + 2 group
+ * tasks are in flight, the graph of the oldest waiting query sends up to
+ * DFQ_SERVE_GROUP_MAX of its waiting queries as one group (flags must be
+ * 0), so queries that arrive while the teams are busy share the next
+ * traversal of their graph.
  * The server must be running.  records[q] receives the timings.  Returns
  * DFQ_ERR_BUSY if an arrival cannot be admitted while none of the replay's
  * own queries is in flight (resources held by unreleased tickets). */

@@ -232,3 +232,68 @@ def test_serve_config5_resident_graphs(dfq, oracle):
             np.testing.assert_array_equal(outs[q], oracle.bfs(g.row_offsets, g.col_idx, int(s2[q])),
                                           err_msg=f"oracle: query {q} graph {g2[q]}")
     say("graphs checked against the oracle:", sorted(seen))
+
+
+# ---- group mode: queries of one graph share traversals --------------------
+
+@pytest.mark.parametrize("xpt,block", [(1, 1024), (1, 512), (2, 1024), (0, 1024)])
+def test_serve_groups_mix_equals_oracle(dfq, serve_setup, xpt, block):
+    """Group-mode server (bit-parallel multi-source group tasks): the
+    Poisson mix of the four graphs whose depth fits the group planes; every
+    query's levels, reach, depth and traversed entries equal the oracle's,
+    and queries did share traversals (fewer tasks than queries)."""
+    gs, gi, so, arrival, expect = serve_setup
+    keep = gi < 4  # the 200 x 150 grid is deeper than 255 levels
+    gi2, so2, ar2 = gi[keep], so[keep], arrival[keep]
+    with dfq.Server(gs, xcds_per_team=xpt, block_size=block, groups=True) as srv:
+        srv.start()
+        say("group server", srv.info())
+        recs, outs = srv.replay(gi2, so2, ar2, keep_levels=True)
+        for q in range(len(gi2)):
+            e = expect[(gi2[q], so2[q])]
+            np.testing.assert_array_equal(outs[q], e, err_msg=f"groups xpt {xpt}: query {q} graph {gi2[q]}")
+            check_record(recs[q], e, gs[gi2[q]])
+        tasks = len(set(zip(recs["admit_us"].tolist(), recs["done_us"].tolist())))
+        say("queries", len(gi2), "group tasks", tasks, "max in flight", max_overlap(recs))
+        assert tasks < len(gi2)
+        srv.stop()
+
+
+def test_serve_groups_direct_api(dfq, oracle, serve_setup):
+    """submit_group: per-member tickets, results and levels; repeated
+    sources; a group of one through submit; release frees the range with
+    the last member; the deep grid is refused with ERR_CAPACITY."""
+    gs, gi, so, arrival, expect = serve_setup
+    g = 1
+    srcs = list(gs[g].pick_sources(40, seed=8)) + [int(gs[g].pick_sources(1, seed=8)[0])] * 3
+    ref = {int(s): oracle.bfs(gs[g].row_offsets, gs[g].col_idx, int(s)) for s in srcs}
+    with dfq.Server(gs, groups=True) as srv:
+        srv.start()
+        tks = srv.submit_group(g, srcs)
+        assert len(tks) == len(srcs) and len(set(tks)) == len(tks)
+        t1 = srv.submit(0, int(gs[0].pick_sources(1, seed=2)[0]))
+        for s, t in zip(srcs, tks):
+            r = srv.wait(t)
+            e = ref[int(s)]
+            assert r.reached == int((e != UNREACHED).sum()) and r.depth == int(e[e != UNREACHED].max()) + 1
+            np.testing.assert_array_equal(srv.levels(t, g), e)
+        used = srv.info()["arena_used"]
+        for t in tks[:-1]:
+            srv.release(t)
+        assert srv.info()["arena_used"] == used  # the last member still holds the range
+        srv.release(tks[-1])
+        assert srv.info()["arena_used"] < used
+        srv.wait(t1)
+        s0 = int(gs[0].pick_sources(1, seed=2)[0])
+        np.testing.assert_array_equal(srv.levels(t1, 0), oracle.bfs(gs[0].row_offsets, gs[0].col_idx, s0))
+        srv.release(t1)
+        # 4: the 200 x 150 grid from a corner: past the 255 levels of the planes
+        t2 = srv.submit(4, 0)
+        with pytest.raises(dfq.DfqError) as e:
+            srv.wait(t2)
+        assert e.value.code == dfq.ERR_CAPACITY
+        srv.release(t2)
+        with pytest.raises(dfq.DfqError):
+            srv.submit(0, 0, dfq.SERVE_PARENTS)  # no parents in group mode
+        assert srv.info()["in_flight"] == 0
+        srv.stop()

@@ -180,6 +180,40 @@ __device__ __forceinline__ uint32_t ld_system(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Request and graph descriptors are read from memory by every thread alike;
+// moved to scalar registers, everything derived from them stays scalar.
+__device__ __forceinline__ ServeRequest uniform_request(const ServeRequest &r) {
+  ServeRequest d;
+  d.seq = uniform(r.seq);
+  d.work = uniform(r.work);
+  d.dst = uniform(r.dst);
+  d.graph = uniform(r.graph);
+  d.source = uniform(r.source);
+  d.flags = uniform(r.flags);
+  d.item_cap = uniform(r.item_cap);
+  d.small_max = uniform(r.small_max);
+  d.medium_max = uniform(r.medium_max);
+  d.large_max = uniform(r.large_max);
+  d.alpha = uniform(r.alpha);
+  d.beta = uniform(r.beta);
+  d.pad = 0;
+  return d;
+}
+
+__device__ __forceinline__ ServeGraph uniform_graph(const ServeGraph &x) {
+  ServeGraph g;
+  g.vbase = uniform(x.vbase);
+  g.ebase = uniform(x.ebase);
+  g.m = uniform(x.m);
+  g.n = uniform(x.n);
+  g.symmetric = uniform(x.symmetric);
+  g.iso_base = uniform(x.iso_base);
+  g.obase = uniform(x.obase);
+  g.nz = uniform(x.nz);
+  g.pad = 0;
+  return g;
+}
+
 // A group task on the team: the multi-source search of up to 64 queries of
 // one graph, then their results into the host-mapped group block and the
 // ring.
@@ -189,7 +223,7 @@ __device__ __forceinline__ void serve_group_task(const ServeArgs &s, const Serve
   ServeCtl *c = s.ctl;
   GridBarrier *bar = &c->team[id].bar;
   const uint32_t tid = threadIdx.x;
-    const ServeGraph g = s.graphs[d.graph];
+    const ServeGraph g = uniform_graph(s.graphs[d.graph]);
     const uint32_t segs = serve_segments(s.xpt);
     const ServeGroupLayout gl = serve_group_layout(g.n, d.source, d.item_cap, segs);
     uint8_t *w = s.arena + d.work;
@@ -311,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     }
     if (grid_barrier(bar, team.bst, stop, sh.bcast)) break;
 
-    const ServeRequest d = c->desc[id];
+    const ServeRequest d = uniform_request(c->desc[id]);
     if (d.flags & kServeCopy) {
       // all workgroups of the team stream the words out (arena ranges and
       // the staging buffer are 256-byte aligned)
@@ -337,7 +371,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
       serve_group_task<BLOCK>(s, d, sh, team, id, t0);
       continue;
     }
-    const ServeGraph g = s.graphs[d.graph];
+    const ServeGraph g = uniform_graph(s.graphs[d.graph]);
     const bool parents = (d.flags & kServeParents) != 0;
     const ServeLayout l = serve_layout(g.n, d.item_cap, serve_segments(s.xpt), parents);
     uint8_t *w = s.arena + d.work;

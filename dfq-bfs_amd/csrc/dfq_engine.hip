@@ -80,6 +80,8 @@ struct dfq_graph {
   uint64_t *d_mscnt = nullptr;
   uint64_t *d_msprof = nullptr;
   uint32_t msprof_levels = 0;
+  uint2 *d_msitems = nullptr;  // work items of multi-source searches, kSegments x msitem_cap
+  uint64_t msitem_cap = 0;
   int grid_ms = 0;
   // degree-ranked copy of the graph for multi-source searches (built at the
   // first one): offsets, columns as ranks (rows sorted), vertex -> rank
@@ -183,6 +185,7 @@ void free_device(dfq_graph *g) {
   (void)hipFree(g->d_msq);
   (void)hipFree(g->d_mscnt);
   (void)hipFree(g->d_msprof);
+  (void)hipFree(g->d_msitems);
   (void)hipFree(g->d_off_r);
   (void)hipFree(g->d_col_r);
   (void)hipFree(g->d_o2r);
@@ -692,6 +695,15 @@ int run_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d
       return DFQ_ERR_HIP;
     }
   }
+  const uint64_t icap = ms_item_cap(g->n, g->m, g->cfg.large_max);
+  if (icap > 0xffffffffull) return DFQ_ERR_CAPACITY;
+  if (icap > g->msitem_cap) {
+    (void)hipFree(g->d_msitems);
+    g->d_msitems = nullptr;
+    g->msitem_cap = 0;
+    HIP_TRY(hipMalloc(&g->d_msitems, sizeof(uint2) * kSegments * icap));
+    g->msitem_cap = icap;
+  }
   uint32_t *d_depth = (uint32_t *)(g->d_mscnt + 2 * kMsMaxSources);
   uint32_t *d_src = d_depth + kMsMaxSources;
   HIP_TRY(hipMemcpyAsync(d_src, sources, sizeof(uint32_t) * count, hipMemcpyHostToDevice, g->stream));
@@ -713,8 +725,8 @@ int run_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d
   a.planes = g->d_ms + 3 * n;
   a.qb = g->d_msq;
   a.segs = kSegments;
-  a.items = g->d_items;
-  a.item_cap = g->item_cap;
+  a.items = g->d_msitems;
+  a.item_cap = (uint32_t)g->msitem_cap;
   a.ctl = g->d_ctl;
   a.counters = g->d_mscnt;
   a.depth = d_depth;

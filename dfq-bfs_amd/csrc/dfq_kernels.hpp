@@ -47,7 +47,8 @@ struct alignas(256) LevelSlot {
   uint32_t item_tail[kSegments];  // work items of level L's grid phase, per segment
   uint64_t active;                // multi-source searches: the searches with a frontier at level L
   uint32_t chunk_next;            // multi-source bottom-up levels: next vertex chunk to claim
-  uint32_t pad[64 - 2 * kSegments - 3];
+  uint32_t item2_tail[kSegments]; // multi-source bottom-up levels: rest-of-row items, per segment
+  uint32_t pad[64 - 3 * kSegments - 3];
 };
 
 // Huge-degree frontier vertices are cut into items of kItem CSR entries:

@@ -158,13 +158,34 @@ struct MsVisitor {
   }
 };
 
-// Rows longer than this are pulled by whole waves in kItem-entry work items
-// in bottom-up levels: a row must be scanned to its end whenever one of the
-// missing searches reaches the vertex later, and one lane on a hub's row
-// would hold up the whole level.
-constexpr uint32_t kMsBuRow = kItem;
-// Items a wave pulls per trip (their loads in flight together).
+// Long rows are not scanned by one lane in bottom-up levels: a row must be
+// read to its end whenever one of the missing searches reaches the vertex
+// later, and one lane on a hub's row would hold up the level.  In the first
+// bottom-up level of a traversal rows rarely finish early, so rows longer
+// than kItem are cut into kItem-entry work items outright.  In later ones
+// most rows find every missing search among their first (highest-degree)
+// neighbours, so a row longer than kMsBuRow gets a probe - one wave reads
+// its first kMsBuRow entries at once - and only a row still missing
+// searches after it is cut into items for the rest.  (Probes of kItem
+// entries in the first level measured slower: RMAT-22 1.71 -> 2.76 ms.)
+constexpr uint32_t kMsBuRow = kWave;
+// Items a wave pulls per trip (their loads in flight together); probes of
+// kMsBuRow entries go kItem / kMsBuRow times as many per trip.
 constexpr int kMsItemBatch = 4;
+
+// Work-item slots per segment a multi-source search needs: top-down levels
+// cut rows longer than large_max into kItem-entry items; a bottom-up level
+// puts one probe per row longer than kMsBuRow at the front of the segment
+// and the rest-of-row items at its back.
+__host__ __device__ inline uint64_t ms_item_cap(uint32_t n, uint64_t m, uint32_t large_max) {
+  uint64_t rows_td = m / ((uint64_t)large_max + 1);
+  if (rows_td > n) rows_td = n;
+  uint64_t rows_bu = m / ((uint64_t)kMsBuRow + 1);
+  if (rows_bu > n) rows_bu = n;
+  const uint64_t td = m / kItem + rows_td + 1;
+  const uint64_t bu = rows_bu + (m / kItem + rows_bu) + 2;
+  return td > bu ? td : bu;
+}
 
 // Bottom-up level of the multi-source search.  Each wave takes chunks of
 // kBuChunk vertices, lists those missing some active search (alive &
@@ -176,12 +197,17 @@ constexpr int kMsItemBatch = 4;
 // nothing to load for a kind read one shared address), so one trip costs
 // one memory round trip.  The searches found become the vertex's next
 // mask, written by its owner lane, and the vertex joins the next queue.
-// Rows longer than kMsBuRow are cut into work items instead (appended to
-// `items` / `item_tail`), pulled by ms_bottomup_items after a barrier.
+// Long rows go elsewhere: with `probe`, rows longer than kMsBuRow get a
+// probe (front of `items`, count in item_tail), run by ms_bottomup_probes
+// after a barrier; otherwise rows longer than kItem are cut into items for
+// the whole row (back of `items`, count in item2_tail), pulled by
+// ms_bottomup_items.
 template <int BLOCK>
 __device__ __forceinline__ void ms_bottomup_level(const MsArgs &a, SharedAll<BLOCK> &sh, MsVisitor &vis,
                                                   uint64_t alive, uint32_t wave, uint32_t *chunk_next,
-                                                  uint2 *items, uint32_t *item_tail) {
+                                                  uint2 *items, uint32_t *item_tail, uint32_t *item2_tail,
+                                                  bool probe) {
+  const int64_t long_len = probe ? (int64_t)kMsBuRow : (int64_t)kItem;
   const uint32_t lane = lane_id();
   const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
   const uint64_t *fcur = vis.fcur;
@@ -251,12 +277,17 @@ __device__ __forceinline__ void ms_bottomup_level(const MsArgs &a, SharedAll<BLO
       const bool fin2 = st2 && (got2 == miss || kn >= e);
       const bool claimed = fin2 && got2 != 0ull;
       const uint64_t miss0 = alive & ~sv;
-      // a long row becomes work items; this lane takes the next candidate
-      const bool long_row = st0 && miss0 != 0ull && o1 - o0 > (int64_t)kMsBuRow;
+      // a long row gets a probe or its items; this lane takes the next candidate
+      const bool long_row = st0 && miss0 != 0ull && o1 - o0 > long_len;
       if (long_row) {
-        const uint32_t nit = (uint32_t)((o1 - o0 + kItem - 1) / kItem);
-        const uint32_t b0 = __hip_atomic_fetch_add(item_tail, nit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (uint32_t j = 0; j < nit; ++j) items[b0 + j] = make_uint2(v, j);
+        if (probe) {
+          const uint32_t b0 = __hip_atomic_fetch_add(item_tail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          items[b0] = make_uint2(v, 0u);
+        } else {
+          const uint32_t nit = (uint32_t)((o1 - o0 + kItem - 1) / kItem);
+          const uint32_t b0 = __hip_atomic_fetch_add(item2_tail, nit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          for (uint32_t j = 0; j < nit; ++j) items[a.item_cap - 1 - (b0 + j)] = make_uint2(v, j);
+        }
       }
       const uint32_t next0 = (o1 > o0 && miss0 != 0ull && !long_row) ? 1u : 0u;
       stage = st0 ? next0 : (st1 ? 2u : (st2 ? (fin2 ? 0u : 1u) : stage));
@@ -277,17 +308,99 @@ __device__ __forceinline__ void ms_bottomup_level(const MsArgs &a, SharedAll<BLO
   }
 }
 
-// The long rows of a bottom-up level: `total` work items located through
-// pre (per-segment item counts), round-robin over every wave of the team.
-// A wave ORs the frontier masks of its item's 256 neighbours, keeps the
-// vertex's missing searches, and merges them into the vertex's next mask
-// with one atomic; the transition from zero appends the vertex to the next
-// queue.  Items of a vertex that already holds every missing search skip
-// their loads (the mask only grows, so a stale copy can only hold fewer).
+// The probes of a bottom-up level's long rows: `total` probes located
+// through pre (per-segment counts, from the front of each segment),
+// round-robin over every wave of the team, several per trip.  A wave ORs
+// the frontier masks of the row's first PL x 64 neighbours, keeps the vertex's missing searches and writes them as its next
+// mask (no other wave touches the vertex in this phase), appending the
+// vertex to the next queue.  A row still missing searches after its probe
+// puts kItem-entry items for the rest at the back of this workgroup's
+// segment (count in item2_tail).
+template <int BLOCK, int PL>
+__device__ __forceinline__ void ms_bottomup_probes(const MsArgs &a, MsVisitor &vis, uint64_t alive,
+                                                   const uint32_t *pre, uint32_t total, uint32_t gwave,
+                                                   uint32_t nwaves, uint2 *seg_items, uint32_t *item2_tail) {
+  // PL loads per lane: probes of PL x 64 entries, 2 x kUnroll loads per lane per trip
+  constexpr int B = 2 * kUnroll / PL;
+  constexpr uint32_t P = (uint32_t)PL * kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
+  for (uint32_t i0 = gwave; i0 < total; i0 += B * nwaves) {
+    uint32_t v[B], len[B], deg[B];
+    int64_t s0[B];
+    uint64_t miss[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const uint32_t i = i0 + (uint32_t)b * nwaves;
+      v[b] = 0;
+      len[b] = 0;
+      deg[b] = 0;
+      s0[b] = 0;
+      miss[b] = 0;
+      if (i < total) {  // wave-uniform
+        uint32_t j = 0;
+#pragma unroll
+        for (int k = 1; k < kSegments; ++k) j += pre[k] <= i;
+        const uint32_t x = a.items[(size_t)j * a.item_cap + (i - pre[j])].x;
+        v[b] = x;
+        s0[b] = a.off[x];
+        deg[b] = (uint32_t)(a.off[x + 1] - s0[b]);
+        len[b] = min(deg[b], P);
+        miss[b] = alive & ~a.seen[x];
+      }
+    }
+    uint32_t c[B][PL];
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        const uint32_t e = (uint32_t)k * kWave + lane;
+        c[b][k] = e < len[b] ? colb[s0[b] + e] : 0u;
+      }
+    uint64_t acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      acc[b] = 0;
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        const uint32_t e = (uint32_t)k * kWave + lane;
+        const uint64_t fw = vis.fcur[c[b][k]];
+        acc[b] |= e < len[b] ? fw : 0ull;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      uint64_t x = acc[b];
+      for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d);
+      const uint64_t nb = x & miss[b];
+      if (lane == 0 && nb) vis.fnxt[v[b]] = nb;
+      const bool claimed = lane == 0 && nb != 0ull;
+      const uint64_t mask = __ballot(claimed);
+      if (mask) vis.append(claimed, mask, v[b]);
+      if (nb != miss[b] && deg[b] > P) {  // wave-uniform
+        const uint32_t nit = (deg[b] - P + kItem - 1) / kItem;
+        uint32_t base = 0;
+        if (lane == 0) base = __hip_atomic_fetch_add(item2_tail, nit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        base = bcast_lane(base, 0);
+        for (uint32_t k = lane; k < nit; k += kWave) seg_items[a.item_cap - 1 - (base + k)] = make_uint2(v[b], k);
+      }
+    }
+  }
+}
+
+// Work items of long rows: `total` items located through pre (per-segment
+// counts, from the back of each segment), round-robin over every wave of
+// the team.  Item (v, k) covers entries first + k * kItem .. of v's row
+// (first = the level's probe length).  A wave ORs the
+// frontier masks of the item's neighbours, keeps the vertex's missing
+// searches, and merges them into the vertex's next mask with one atomic;
+// the transition from zero appends the vertex to the next queue.  Items of
+// a vertex that already holds every missing search skip their loads (the
+// mask only grows, so a stale copy can only hold fewer).
 template <int BLOCK>
 __device__ __forceinline__ void ms_bottomup_items(const MsArgs &a, MsVisitor &vis, uint64_t alive,
                                                   const uint32_t *pre, uint32_t total, uint32_t gwave,
-                                                  uint32_t nwaves) {
+                                                  uint32_t nwaves, uint32_t first) {
   constexpr int B = kMsItemBatch;
   const uint32_t lane = lane_id();
   const uint32_t *colb = a.col_bu ? a.col_bu : a.col;
@@ -309,9 +422,9 @@ __device__ __forceinline__ void ms_bottomup_items(const MsArgs &a, MsVisitor &vi
         uint32_t j = 0;
 #pragma unroll
         for (int k = 1; k < kSegments; ++k) j += pre[k] <= i;
-        const uint2 it = a.items[(size_t)j * a.item_cap + (i - pre[j])];
+        const uint2 it = a.items[(size_t)j * a.item_cap + a.item_cap - 1 - (i - pre[j])];
         v[b] = it.x;
-        s0[b] = a.off[it.x] + (int64_t)it.y * kItem;
+        s0[b] = a.off[it.x] + first + (int64_t)it.y * kItem;
         len[b] = (uint32_t)min((int64_t)kItem, a.off[it.x + 1] - s0[b]);
         miss[b] = alive & ~a.seen[it.x];
         have[b] = vis.fnxt[it.x];
@@ -425,6 +538,7 @@ __device__ __forceinline__ void ms_grid_search(const MsArgs &a, SharedAll<BLOCK>
     for (int k = 0; k < kSegments; ++k) {
       sl.tail[k] = 0;
       sl.item_tail[k] = 0;
+      sl.item2_tail[k] = 0;
     }
     sl.active = 0ull;
     sl.chunk_next = 0;
@@ -494,10 +608,11 @@ __device__ __forceinline__ void ms_grid_search(const MsArgs &a, SharedAll<BLOCK>
     }
     // the previous level's masks leave the buffer the next level fills
     for (uint32_t i = gtid; i < P; i += gstride) fnxt[ms_queue_at(prev_q, ms_pre, n, i)] = 0ull;
-    if (gtid < 2 * kSegments) {
+    if (gtid < 3 * kSegments) {
       LevelSlot &z = ctl->slot[(L + 1) & 3];
       if (gtid < kSegments) z.tail[gtid] = 0;
-      else z.item_tail[gtid - kSegments] = 0;
+      else if (gtid < 2 * kSegments) z.item_tail[gtid - kSegments] = 0;
+      else z.item2_tail[gtid - 2 * kSegments] = 0;
       if (gtid == 0) {
         z.active = 0ull;
         z.chunk_next = 0;
@@ -550,19 +665,38 @@ __device__ __forceinline__ void ms_grid_search(const MsArgs &a, SharedAll<BLOCK>
       }
     } else {
       ++bu_levels;
-      ms_bottomup_level<BLOCK>(a, sh, vis, alive, wave, &cur.chunk_next, a.items + (size_t)sh.seg * a.item_cap,
-                               &cur.item_tail[sh.seg]);
+      uint2 *seg_items = a.items + (size_t)sh.seg * a.item_cap;
+      const bool probe = bu_levels > 1;
+      ms_bottomup_level<BLOCK>(a, sh, vis, alive, wave, &cur.chunk_next, seg_items, &cur.item_tail[sh.seg],
+                               &cur.item2_tail[sh.seg], probe);
       const uint32_t wg = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       next = (uint32_t)grid_barrier(&ctl->bar, bst, wg, sh.bcast);
-      if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
+      if (tid < kWave) tails_prefix<kSegments>(probe ? cur.item_tail : cur.item2_tail, sh.item_pre);
       __syncthreads();
-      const uint32_t hc = uniform(sh.item_pre[kSegments]);
-      items = hc;
-      if (hc) {
+      const uint32_t hp = uniform(sh.item_pre[kSegments]);
+      items = hp;
+      if (hp && !probe) {
         const uint32_t before = vis.claims;
-        ms_bottomup_items<BLOCK>(a, vis, alive, sh.item_pre, hc, gwave, nwaves);
+        ms_bottomup_items<BLOCK>(a, vis, alive, sh.item_pre, hp, gwave, nwaves, 0u);
+        const uint32_t wg3 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
+        next += (uint32_t)grid_barrier(&ctl->bar, bst, wg3, sh.bcast);
+      } else if (hp) {
+        // probes of the long rows, then the rest of the rows they left open
+        uint32_t before = vis.claims;
+        ms_bottomup_probes<BLOCK, 1>(a, vis, alive, sh.item_pre, hp, gwave, nwaves, seg_items,
+                                     &cur.item2_tail[sh.seg]);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
         next += (uint32_t)grid_barrier(&ctl->bar, bst, wg2, sh.bcast);
+        if (tid < kWave) tails_prefix<kSegments>(cur.item2_tail, sh.item_pre);
+        __syncthreads();
+        const uint32_t hc = uniform(sh.item_pre[kSegments]);
+        items += hc;
+        if (hc) {
+          before = vis.claims;
+          ms_bottomup_items<BLOCK>(a, vis, alive, sh.item_pre, hc, gwave, nwaves, kMsBuRow);
+          const uint32_t wg3 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
+          next += (uint32_t)grid_barrier(&ctl->bar, bst, wg3, sh.bcast);
+        }
       }
     }
     if (a.prof && trank == 0 && tid == 0 && L < a.prof_levels) {

@@ -175,10 +175,9 @@ struct dfq_server {
 
 namespace {
 
-// Work items a group task's levels may create (top-down rows above
-// large_max, bottom-up rows above kMsBuRow), per segment.
+// Work-item slots per segment a group task's levels may use.
 uint32_t group_item_cap(const ServeGraph &g, const dfq_query_plan &p) {
-  return serve_item_cap(g.n, g.m, std::min<uint32_t>(p.large_max, kMsBuRow));
+  return (uint32_t)std::min<uint64_t>(ms_item_cap(g.n, g.m, p.large_max), 0xffffffffull);
 }
 
 uint64_t group_footprint(const ServeGraph &g, const dfq_query_plan &p, uint32_t xpt, uint32_t count) {

@@ -420,6 +420,9 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
       b.results[qi].bu_levels = b.grid.ctl->bu_levels;
     }
   }
+  // the grid tier's level barriers leave caches as they are (grid_bfs
+  // invalidates before its own shared reads only)
+  if (b.n_grid && threadIdx.x == 0) l2_invalidate();
 
   // workgroup tier: instances from the shared counter, one per workgroup
   uint32_t *qbuf = b.wg_queue + (size_t)blockIdx.x * 2 * kWgMaxN;

@@ -344,6 +344,13 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
 // thread 0.  `s_word` points at two 64-bit LDS slots.
 // kSystem: the XCD leaders write their L2 back at system scope, so every
 // store made before the barrier can be read by the host or a DMA copy.
+// kInv = false: the workgroup does not invalidate its caches on release;
+// the caller must invalidate (make_fresh) before it reads, with plain
+// loads, data other workgroups wrote.  An invalidate acts on the XCD's L2
+// as well as the CU's L1 (a device-scope buffer_inv is the only one that
+// drops L1 lines on gfx950), and the workgroups of one XCD invalidating
+// together wait for each other: 32 of them take ~2 us.  Levels where few
+// workgroups have work skip most of that.
 //
 // The payload and the side sum travel in two arrival words each carrying
 // its own count, added together by every workgroup (one round trip).  Per
@@ -352,7 +359,7 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
 // its sum to the root's side word (the two may be different workgroups).
 // The last root arrivals publish generation | total in gensum and
 // side_out; every workgroup polls both words in one round trip per poll.
-template <bool kSystem = false>
+template <bool kSystem = false, bool kInv = true>
 __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &st, uint64_t payload,
                                                  uint64_t *s_word, uint64_t side = 0,
                                                  uint64_t *side_total = nullptr) {
@@ -413,7 +420,7 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
     // this CU's L1 may hold stale lines; on gfx950 only the device-scope
     // invalidate reaches it (workgroup scope leaves L1 alone outside
     // thread-group-split mode), and it also drops this XCD's L2 lines
-    l2_invalidate();
+    if (kInv) l2_invalidate();
     st.gen = next;
     s_word[0] = w & kPayloadMask;
     s_word[1] = ws & kPayloadMask;
@@ -421,6 +428,18 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
   __syncthreads();
   if (with_side) *side_total = uniform(s_word[1]);
   return uniform(s_word[0]);
+}
+
+// Lazy invalidation after a grid_barrier<.., false>: a workgroup about to
+// read (with plain loads) data that other workgroups wrote since the
+// barrier invalidates first, once per barrier.  Every thread of the
+// workgroup calls it with the same `need`.
+__device__ __forceinline__ void make_fresh(bool &fresh, bool need) {
+  if (need && !fresh) {
+    if (threadIdx.x == 0) l2_invalidate();
+    __syncthreads();
+    fresh = true;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -443,6 +462,7 @@ struct Visitor {
   uint32_t *vis;
   const uint32_t *snap; // visited bitmap as of the level's start (subset of vis), or vis itself
   bool probe_live;      // snap is a snapshot: probe the live bitmap after it
+  bool direct;          // claim by the atomic alone, no bitmap probe before it
   uint32_t *seg;        // output segment of this workgroup's XCD
   uint32_t *seg_tail;   // its tail
   uint32_t *stage;      // this wave's LDS staging buffer [kStage]
@@ -498,8 +518,13 @@ struct Visitor {
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
                                               const uint32_t (&w)[U]) {
     uint32_t word[U];
+    if (direct) {
 #pragma unroll
-    for (int i = 0; i < U; ++i) word[i] = act[i] ? snap[w[i] >> 5] : 0xffffffffu;
+      for (int i = 0; i < U; ++i) word[i] = act[i] ? 0u : 0xffffffffu;
+    } else {
+#pragma unroll
+      for (int i = 0; i < U; ++i) word[i] = act[i] ? snap[w[i] >> 5] : 0xffffffffu;
+    }
     if (probe_live) {
 #pragma unroll
       for (int i = 0; i < U; ++i)

@@ -383,7 +383,8 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 template <int BLOCK, bool kProf, class V>
 __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
-                                uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
+                                uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges,
+                                bool *fresh = nullptr) {
 
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
@@ -400,6 +401,8 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
   const uint32_t nblk = (total + kWave - 1) / kWave;
   const uint32_t G = tsize, W = (uint32_t)S::kWaves;
   const uint32_t mine = nblk > trank ? (nblk - trank - 1) / G + 1 : 0;
+  // only a workgroup with frontier blocks reads shared data
+  if (fresh) make_fresh(*fresh, mine != 0);
   {
     for (uint32_t c0 = 0; c0 < mine; c0 += W) {
       const uint32_t c = c0 + wave;  // this wave's block ordinal within the workgroup
@@ -736,6 +739,13 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, const SearchIo 
 // least N / kSnapshotDiv out-edges.
 constexpr uint64_t kSnapshotDiv = 16;
 
+// A top-down level without a snapshot whose frontier has at most this many
+// out-edges claims by the atomic alone (Visitor::direct): the probe before
+// it is one more wait on a level that is all latency, and the extra atomics
+// are few (4096^2 grid 11.7 -> 11.3 us per level, RMAT-22 hybrid 159.8 ->
+// 157.9 us per search, measured).
+constexpr uint64_t kDirectMaxEdges = 65536;
+
 // Solo levels: at most this many frontier vertices and out-edges.
 constexpr uint64_t kSoloMaxFrontier = 1024;
 constexpr uint64_t kSoloMaxEdges = 8192;
@@ -893,7 +903,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   } else {
     if (tid == 0) solo_next = (ld_relaxed(&ctl->solo_rel.word) & ~kSoloDone) + 1;
     bfs_init(a, io, gtid, gstride);
-    grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+    grid_barrier<false, false>(&ctl->bar, bst, 0, sh.bcast);
   }
   const size_t fbs = (size_t)a.n_words + 1;  // words per frontier bitmap
 
@@ -904,6 +914,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   vis.vis = a.vis;
   vis.snap = a.vis;
   vis.probe_live = false;
+  vis.direct = false;
   vis.stage = sh.stage[wave];
   vis.out_n = 0;
   vis.probes = 0;
@@ -938,6 +949,9 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   // state (or the end of the search).
   bool solo = false;
   uint64_t solo_tag = 0;
+  // The level barriers leave caches as they are; `fresh`: this workgroup
+  // has invalidated since the last one (make_fresh before shared reads).
+  bool fresh = mode.pre_init;
   for (uint32_t L = 0;; ++L) {
     // selects, not a[L & 1]: an array member indexed at run time would put
     // the whole argument block in scratch memory
@@ -970,6 +984,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         // solo levels leave the frontier bitmaps alone (fb_zero holds) but
         // do not build the next one
         fb_valid = false;
+        fresh = true;  // solo_wait invalidated
         --L;  // the loop's ++L gives the level to resume at
         continue;
       }
@@ -997,8 +1012,11 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n;
       vis.snap = snapshot ? a.vis_snap : a.vis;
       vis.probe_live = snapshot;
-      if (snapshot)
+      vis.direct = !snapshot && m_f <= kDirectMaxEdges;
+      if (snapshot) {
+        make_fresh(fresh, true);
         for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
+      }
       // A growing frontier may turn the next level bottom-up: zero this
       // level's bitmap parity for later and OR the claims into the next
       // one when it is known to be zero.  Solo levels and shrinking
@@ -1014,18 +1032,21 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the
       // team in one pass; otherwise degree bins
-      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull)
+      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull) {
+        make_fresh(fresh, true);  // every workgroup reads the whole frontier
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
-      else
+      } else {
         topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
-                               edges);
+                               edges, &fresh);
+      }
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
       uint64_t mf1 = 0;
       if (solo) {
         n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
       } else {
-        n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+        n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+        fresh = false;
       }
       m_f = mf1;
       if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
@@ -1035,14 +1056,18 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         const uint32_t before = vis.claims;
         uint64_t th = 0;
         if (kProf && a.prof && tid == 0) th = now_ns();
+        // a workgroup reads work items only when its first wave has one
+        make_fresh(fresh, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
         topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
         if (kProf && a.prof && tid == 0) busy += now_ns() - th;
         uint64_t mf2 = 0;
-        if (solo)
+        if (solo) {
           n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
-        else
-          n_next += grid_barrier(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
+        } else {
+          n_next += grid_barrier<false, false>(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
+          fresh = false;
+        }
         m_f += mf2;
       }
     } else {
@@ -1050,9 +1075,12 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       uint32_t *fb_cur = a.fb + (L & 1) * fbs;
       uint32_t *fb_nxt = a.fb + ((L + 1) & 1) * fbs;
       if (!fb_valid) {
+        make_fresh(fresh, true);
         build_frontier_bitmap(a, io.level, L, fb_cur, gwave, nwaves);
-        grid_barrier(&ctl->bar, bst, 0, sh.bcast);
+        grid_barrier<false, false>(&ctl->bar, bst, 0, sh.bcast);
+        fresh = false;
       }
+      make_fresh(fresh, true);
       vis.fb_mark = nullptr;
       fb_zero = 0;
       fb_valid = true;
@@ -1062,7 +1090,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       if (kProf && a.prof && tid == 0) busy += now_ns() - tb;
       uint64_t mf1 = 0;
-      n_next = grid_barrier(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      fresh = false;
       m_f = mf1;
     }
     m_seen += m_f;

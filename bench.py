@@ -269,6 +269,44 @@ def batch_on(args, dfq, b, graphs, gi, so):
     return out
 
 
+def run_levels(args, dfq):
+    """Per-level counters of the profiling kernel (dfq_bfs_profile): every
+    level of the first --level-sources searches of the headline graph in
+    --mode, each next to the plain launch's time, and the 4096^2 grid's
+    levels summarised by frontier size (the profiling kernel adds per-level
+    atomics and timestamps, so its times run above the plain launch's)."""
+    out = {"mode": args.mode, "rmat": [], "grid": {}}
+    csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)
+    srcs = csr.pick_sources(args.sources, seed=2022)
+    keep = ("frontier", "edges", "probes", "atomics", "claims", "items", "bottom_up", "level_ns",
+            "busy_max_ns", "bin_small", "bin_medium", "bin_huge", "bu_candidates")
+    with dfq.DeviceGraph(csr) as g:
+        g.set_config(**MODES[args.mode])
+        for s in srcs[:args.level_sources]:
+            g.bfs(int(s))
+            plain = g.bfs(int(s))
+            rows, st = g.profile(int(s))
+            out["rmat"].append({"source": int(s), "plain_ms": round(plain.kernel_ms, 4),
+                                "profiled_ms": round(st.kernel_ms, 4),
+                                "levels": [{k: r[k] for k in keep} for r in rows]})
+    c3 = dfq.Csr.grid2d(4096, 4096, 0.02, 4096)
+    with dfq.DeviceGraph(c3) as g:
+        g.set_config(direction_opt=0)
+        g.bfs(0)
+        plain = g.bfs(0)
+        rows, st = g.profile(0, max_levels=8192)
+        fr = np.array([r["frontier"] for r in rows])
+        ns = np.array([r["level_ns"] for r in rows], dtype=np.float64)
+        out["grid"] = {"plain_ms": round(plain.kernel_ms, 3), "profiled_ms": round(st.kernel_ms, 3),
+                       "levels": len(rows), "by_frontier": []}
+        for lo, hi in [(0, 64), (64, 256), (256, 1024), (1024, 2048), (2048, 4096), (4096, 1 << 30)]:
+            m = (fr >= lo) & (fr < hi)
+            if m.any():
+                out["grid"]["by_frontier"].append({"frontier": [lo, hi], "levels": int(m.sum()),
+                                                   "profiled_us_mean": round(float(ns[m].mean()) / 1e3, 2)})
+    return out
+
+
 SERVE_SCALES = (18, 19, 20, 21, 22, 22, 23, 24)
 
 
@@ -433,9 +471,12 @@ def main():
                     help="skip the batched and served objects of the headline line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy of BYTES (counter calibration)")
-    ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "batch", "serve", "levels"], default="plain",
                     help="plain: the RMAT-22 headline; batch: batched queries (one launch); "
-                         "serve: resident graphs under Poisson arrivals")
+                         "serve: resident graphs under Poisson arrivals; levels: per-level "
+                         "counters (dfq_bfs_profile) of --level-sources RMAT sources in --mode "
+                         "and of the 4096^2 grid")
+    ap.add_argument("--level-sources", type=int, default=2)
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--batch-sources", type=int, default=4)
     ap.add_argument("--serve-queries", type=int, default=100000)
@@ -474,6 +515,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args, dfq)), flush=True)
+        return
+    if args.workload == "levels":
+        print(json.dumps(run_levels(args, dfq)), flush=True)
         return
     t0 = time.perf_counter()
     csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)

@@ -345,7 +345,7 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
 // kSystem: the XCD leaders write their L2 back at system scope, so every
 // store made before the barrier can be read by the host or a DMA copy.
 // kInv = false: the workgroup does not invalidate its caches on release;
-// the caller must invalidate (make_fresh) before it reads, with plain
+// the caller must invalidate (l2_invalidate) before it reads, with plain
 // loads, data other workgroups wrote.  An invalidate acts on the XCD's L2
 // as well as the CU's L1 (a device-scope buffer_inv is the only one that
 // drops L1 lines on gfx950), and the workgroups of one XCD invalidating
@@ -428,18 +428,6 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
   __syncthreads();
   if (with_side) *side_total = uniform(s_word[1]);
   return uniform(s_word[0]);
-}
-
-// Lazy invalidation after a grid_barrier<.., false>: a workgroup about to
-// read (with plain loads) data that other workgroups wrote since the
-// barrier invalidates first, once per barrier.  Every thread of the
-// workgroup calls it with the same `need`.
-__device__ __forceinline__ void make_fresh(bool &fresh, bool need) {
-  if (need && !fresh) {
-    if (threadIdx.x == 0) l2_invalidate();
-    __syncthreads();
-    fresh = true;
-  }
 }
 
 // ---------------------------------------------------------------------------

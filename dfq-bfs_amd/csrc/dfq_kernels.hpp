@@ -380,11 +380,13 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 
 // Top-down phase A over this workgroup's equal share of the frontier.
 // kProf: the per-level counters are compiled in (the profiling kernel only).
+// Shared data is first read after the workgroup barrier that follows the
+// segment tails, so an invalidate by thread 0 before the call is ordered
+// before those reads.
 template <int BLOCK, bool kProf, class V>
 __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
-                                uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges,
-                                bool *fresh = nullptr) {
+                                uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
 
   using S = KernelShape<BLOCK>;
   const uint32_t tid = threadIdx.x;
@@ -401,8 +403,6 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
   const uint32_t nblk = (total + kWave - 1) / kWave;
   const uint32_t G = tsize, W = (uint32_t)S::kWaves;
   const uint32_t mine = nblk > trank ? (nblk - trank - 1) / G + 1 : 0;
-  // only a workgroup with frontier blocks reads shared data
-  if (fresh) make_fresh(*fresh, mine != 0);
   {
     for (uint32_t c0 = 0; c0 < mine; c0 += W) {
       const uint32_t c = c0 + wave;  // this wave's block ordinal within the workgroup
@@ -735,6 +735,19 @@ __device__ __forceinline__ void bottomup_level(const BfsArgs &a, const SearchIo 
   }
 }
 
+// Lazy invalidation after grid_barrier<.., false>: a workgroup about to
+// read (with plain loads) data that other workgroups wrote since the
+// barrier invalidates first, once per barrier; the flag is bit kFresh of a
+// state word.  Every thread of the workgroup calls it with the same `need`.
+constexpr uint32_t kFresh = 4;
+__device__ __forceinline__ void fresh_bit(uint32_t &word, bool need) {
+  if (need && !(word & kFresh)) {
+    if (threadIdx.x == 0) l2_invalidate();
+    __syncthreads();
+    word |= kFresh;
+  }
+}
+
 // A top-down level takes a visited-bitmap snapshot when its frontier has at
 // least N / kSnapshotDiv out-edges.
 constexpr uint64_t kSnapshotDiv = 16;
@@ -941,7 +954,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   // there) and, when fb[(L + 1) & 1] is known zero, ORs its claims into it,
   // so a bottom-up level right after it needs no level-array scan; a
   // bottom-up level writes the next bitmap whole.
-  uint32_t fb_zero = 3;
+  // bit kFresh of the same word: this workgroup has invalidated its caches
+  // since the last level barrier (those leave caches as they are; see
+  // fresh_bit), kept here rather than in a register of its own
+  uint32_t fb_zero = 3u | (mode.pre_init ? kFresh : 0u);
   bool fb_valid = false;
   // Solo levels: a small top-down level (and the small ones after it) is
   // run by the team's first workgroup alone, separated by workgroup
@@ -949,9 +965,6 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   // state (or the end of the search).
   bool solo = false;
   uint64_t solo_tag = 0;
-  // The level barriers leave caches as they are; `fresh`: this workgroup
-  // has invalidated since the last one (make_fresh before shared reads).
-  bool fresh = mode.pre_init;
   for (uint32_t L = 0;; ++L) {
     // selects, not a[L & 1]: an array member indexed at run time would put
     // the whole argument block in scratch memory
@@ -984,7 +997,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         // solo levels leave the frontier bitmaps alone (fb_zero holds) but
         // do not build the next one
         fb_valid = false;
-        fresh = true;  // solo_wait invalidated
+        fb_zero |= kFresh;  // solo_wait invalidated
         --L;  // the loop's ++L gives the level to resume at
         continue;
       }
@@ -1014,7 +1027,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       vis.probe_live = snapshot;
       vis.direct = !snapshot && m_f <= kDirectMaxEdges;
       if (snapshot) {
-        make_fresh(fresh, true);
+        fresh_bit(fb_zero, true);
         for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];
       }
       // A growing frontier may turn the next level bottom-up: zero this
@@ -1027,17 +1040,23 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
         fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
         vis.fb_mark = fb_track ? a.fb + ((L + 1) & 1) * fbs : nullptr;
-        fb_zero = 1u << (L & 1);
+        fb_zero = (fb_zero & kFresh) | (1u << (L & 1));
       }
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the
       // team in one pass; otherwise degree bins
       if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull) {
-        make_fresh(fresh, true);  // every workgroup reads the whole frontier
+        fresh_bit(fb_zero, true);  // every workgroup reads the whole frontier
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
       } else {
-        topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, solo ? 0u : trank, solo ? 1u : tsize,
-                               edges, &fresh);
+        // block b of the frontier goes to rank b mod the team size: only
+        // ranks below the block count read shared data
+        const uint32_t ra = solo ? 0u : trank;
+        if (!(fb_zero & kFresh) && (uint64_t)ra * kWave < n_cur) {
+          if (tid == 0) l2_invalidate();
+          fb_zero |= kFresh;
+        }
+        topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
       }
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
@@ -1046,7 +1065,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
       } else {
         n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
-        fresh = false;
+        fb_zero &= ~kFresh;
       }
       m_f = mf1;
       if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
@@ -1057,7 +1076,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         uint64_t th = 0;
         if (kProf && a.prof && tid == 0) th = now_ns();
         // a workgroup reads work items only when its first wave has one
-        make_fresh(fresh, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
+        fresh_bit(fb_zero, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
         topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
         if (kProf && a.prof && tid == 0) busy += now_ns() - th;
@@ -1066,7 +1085,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
           n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
         } else {
           n_next += grid_barrier<false, false>(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
-          fresh = false;
+          fb_zero &= ~kFresh;
         }
         m_f += mf2;
       }
@@ -1075,14 +1094,14 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       uint32_t *fb_cur = a.fb + (L & 1) * fbs;
       uint32_t *fb_nxt = a.fb + ((L + 1) & 1) * fbs;
       if (!fb_valid) {
-        make_fresh(fresh, true);
+        fresh_bit(fb_zero, true);
         build_frontier_bitmap(a, io.level, L, fb_cur, gwave, nwaves);
         grid_barrier<false, false>(&ctl->bar, bst, 0, sh.bcast);
-        fresh = false;
+        fb_zero &= ~kFresh;
       }
-      make_fresh(fresh, true);
+      fresh_bit(fb_zero, true);
       vis.fb_mark = nullptr;
-      fb_zero = 0;
+      fb_zero &= kFresh;
       fb_valid = true;
       uint64_t tb = 0;
       if (kProf && a.prof && tid == 0) tb = now_ns();
@@ -1091,7 +1110,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       if (kProf && a.prof && tid == 0) busy += now_ns() - tb;
       uint64_t mf1 = 0;
       n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
-      fresh = false;
+      fb_zero &= ~kFresh;
       m_f = mf1;
     }
     m_seen += m_f;

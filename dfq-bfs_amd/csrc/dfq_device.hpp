@@ -67,24 +67,36 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
   return x;
 }
 
+// Sum over the 64 lanes of a value below 2^40 (the barrier's sums are
+// 40-bit fields), in a scalar register: DPP scans of its two 20-bit
+// halves (neither half's sum can overflow 32 bits), no LDS permutes.
+// Every lane of the wave must be active.
+__device__ __forceinline__ uint64_t wave_sum_u40(uint64_t v) {
+  const uint32_t a = wave_inclusive_scan((uint32_t)v & 0xfffffu);
+  const uint32_t b = wave_inclusive_scan((uint32_t)(v >> 20) & 0xfffffu);
+  return (uint64_t)bcast_lane(a, 63) + ((uint64_t)bcast_lane(b, 63) << 20);
+}
+
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Exclusive prefix of the kSegments tails into pre[0..kSegments], the total
 // in pre[kSegments].  Wave 0 only: lanes 0..kSegments-1 load one tail each,
-// so the tails cost one memory round trip instead of a chain of them.
+// so the tails cost one memory round trip instead of a chain of them; the
+// prefix is row_shr DPP steps inside the first 16-lane row.
 template <int kN>
 __device__ __forceinline__ void tails_prefix(const uint32_t *tails, uint32_t *pre) {
-  static_assert(kN <= kWave && (kN & (kN - 1)) == 0, "tail count");
+  static_assert(kN <= 16 && (kN & (kN - 1)) == 0, "tail count");
   const uint32_t lane = lane_id();
   const uint32_t t = lane < (uint32_t)kN ? ld_agent(&tails[lane]) : 0u;
   uint32_t x = t;
-#pragma unroll
-  for (int d = 1; d < kN; d <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, d);
-    if (lane >= (uint32_t)d) x += y;
-  }
+  // lanes with no source in the row get 0; lanes >= kN hold t = 0, so the
+  // sums of lanes 0..kN-1 are their inclusive prefixes
+  if (kN > 1) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  if (kN > 2) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  if (kN > 4) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  if (kN > 8) x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
   if (lane < (uint32_t)kN) pre[lane] = x - t;
   if (lane == (uint32_t)kN - 1) pre[kN] = x;
 }

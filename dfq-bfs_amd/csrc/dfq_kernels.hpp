@@ -194,9 +194,9 @@ struct SharedAll {
   uint32_t seg;  // output segment of this workgroup (its XCD's rank in the team)
   uint32_t team_rank, team_size, team_id;
   uint32_t wave_sum[KernelShape<BLOCK>::kWaves];
+  uint64_t wave_mf[KernelShape<BLOCK>::kWaves];
   uint64_t prof[kProfFields];
   uint64_t bcast[2];
-  uint64_t mf_sum;
   uint32_t med_n, big_n;
   uint32_t census[2 * kMaxXcd];  // kernel start: workgroups per XCD rank, team generations
 };
@@ -313,18 +313,20 @@ __device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, ui
   return sum;
 }
 
-// Workgroup sum of the per-lane out-degree sums of claimed vertices (thread
-// 0 gets it); resets the lanes' counters.  Ends with a __syncthreads.
+// Workgroup sum of the per-lane out-degree sums of claimed vertices, in
+// every thread; resets the lanes' counters.  Per wave a DPP sum (no LDS
+// permutes), then the wave sums through LDS.  Ends with a __syncthreads.
 template <int BLOCK, class V>
 __device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, V &vis) {
-  if (threadIdx.x == 0) sh.mf_sum = 0;
-  __syncthreads();
-  uint64_t v = vis.mf;
+  constexpr int W = KernelShape<BLOCK>::kWaves;
+  const uint64_t w = wave_sum_u40(vis.mf);
   vis.mf = 0;
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-  if (lane_id() == 0 && v) atomicAdd((unsigned long long *)&sh.mf_sum, (unsigned long long)v);
+  if (lane_id() == 0) sh.wave_mf[threadIdx.x / kWave] = w;
   __syncthreads();
-  return uniform(sh.mf_sum);
+  uint64_t t = 0;
+  for (int i = 0; i < W; ++i) t += sh.wave_mf[i];
+  __syncthreads();  // the slots are free again
+  return uniform(t);
 }
 
 // Sum over all lanes of the workgroup of a per-lane value into sh.prof[f]
@@ -769,9 +771,12 @@ template <int BLOCK>
 __device__ __forceinline__ uint64_t solo_sync(SharedAll<BLOCK> &sh, uint32_t claims, uint64_t mf,
                                               uint64_t *mf_out) {
   wait_own_stores();
-  if (threadIdx.x == 0) sh.bcast[0] = claims;
+  if (threadIdx.x == 0) {
+    sh.bcast[0] = claims;
+    sh.bcast[1] = mf;
+  }
   __syncthreads();
-  *mf_out = mf;
+  *mf_out = uniform(sh.bcast[1]);
   const uint64_t n = uniform(sh.bcast[0]);
   __syncthreads();
   return n;
@@ -1059,12 +1064,13 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
       }
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
+      const uint64_t wg_mf = block_sum_mf<BLOCK>(sh, vis);
       if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
       uint64_t mf1 = 0;
       if (solo) {
-        n_next = solo_sync<BLOCK>(sh, wg_claims, block_sum_mf<BLOCK>(sh, vis), &mf1);
+        n_next = solo_sync<BLOCK>(sh, wg_claims, wg_mf, &mf1);
       } else {
-        n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+        n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, wg_mf, &mf1);
         fb_zero &= ~kFresh;
       }
       m_f = mf1;
@@ -1079,12 +1085,13 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         fresh_bit(fb_zero, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
         topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
+        const uint64_t wg_mf2 = block_sum_mf<BLOCK>(sh, vis);
         if (kProf && a.prof && tid == 0) busy += now_ns() - th;
         uint64_t mf2 = 0;
         if (solo) {
-          n_next += solo_sync<BLOCK>(sh, wg2, block_sum_mf<BLOCK>(sh, vis), &mf2);
+          n_next += solo_sync<BLOCK>(sh, wg2, wg_mf2, &mf2);
         } else {
-          n_next += grid_barrier<false, false>(&ctl->bar, bst, wg2, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf2);
+          n_next += grid_barrier<false, false>(&ctl->bar, bst, wg2, sh.bcast, wg_mf2, &mf2);
           fb_zero &= ~kFresh;
         }
         m_f += mf2;
@@ -1107,9 +1114,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       if (kProf && a.prof && tid == 0) tb = now_ns();
       bottomup_level<BLOCK, kProf>(a, io, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
+      const uint64_t wg_mf = block_sum_mf<BLOCK>(sh, vis);
       if (kProf && a.prof && tid == 0) busy += now_ns() - tb;
       uint64_t mf1 = 0;
-      n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &mf1);
+      n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, wg_mf, &mf1);
       fb_zero &= ~kFresh;
       m_f = mf1;
     }

@@ -761,9 +761,13 @@ constexpr uint64_t kSnapshotDiv = 16;
 // 157.9 us per search, measured).
 constexpr uint64_t kDirectMaxEdges = 65536;
 
-// Solo levels: at most this many frontier vertices and out-edges.
-constexpr uint64_t kSoloMaxFrontier = 1024;
-constexpr uint64_t kSoloMaxEdges = 8192;
+// Solo levels: at most this many frontier vertices and out-edges.  Once
+// team levels stopped invalidating every workgroup's caches, one
+// workgroup's chain grew costlier than a team level's beyond ~512 vertices
+// (4096^2 grid: 1,024 / 8,192 -> 512 / 4,096 took 10.9 -> 10.4 us per
+// level; 256 / 2,048 the same, 64 / 512 10.5; RMAT-22 156.0 -> 154.9 us).
+constexpr uint64_t kSoloMaxFrontier = 512;
+constexpr uint64_t kSoloMaxEdges = 4096;
 
 // End of a solo level: the workgroup's claims and out-degree sum, and its
 // stores made visible to all its waves (same CU, so no cache maintenance).

@@ -182,15 +182,15 @@ def test_serve_restart_and_results_after_stop(dfq, serve_setup):
             assert srv.info()["in_flight"] == 0
 
 
-def test_serve_config5_resident_graphs(dfq, oracle):
+@pytest.fixture(scope="module")
+def config5_setup(dfq):
     """Config 5 graphs at their stated sizes, built exactly as bench.py
     builds them: 8 resident undirected RMAT graphs of scales 18, 19, 20, 21,
     22, 22, 23, 24 (edgefactor 16), queries picking graph i with p ~ 1/scale_i
-    and a uniform degree>0 source.  3,000 Poisson arrivals at about half the
-    served rate (many queries in flight): every query's reached count and
-    depth equal the plain engine's; then 48 queries served in a dense mix
-    give full level arrays equal to the plain engine's, and one per graph
-    equal to the oracle's."""
+    and a uniform degree>0 source.  A trace of 3,000 Poisson arrivals at
+    about half the served rate (many queries in flight) and a dense sample of
+    48 queries, with the plain engine's results: reached count and depth of
+    every query of the trace, full level arrays of the sample."""
     import bench
     scales = bench.SERVE_SCALES
     gs = [dfq.Csr.rmat(s, 16, 0.57, 0.19, 0.19, seed=5000 + i) for i, s in enumerate(scales)]
@@ -211,6 +211,14 @@ def test_serve_config5_resident_graphs(dfq, oracle):
                 for q in np.flatnonzero(g2 == g):
                     plain_lv[q] = dg.bfs(int(s2[q])).level
     assert all(lv is not None for lv in plain_lv)
+    return gs, (gi, so, arr), (g2, s2, arr2), plain, plain_lv
+
+
+def test_serve_config5_resident_graphs(dfq, oracle, config5_setup):
+    """Config 5, one search per task: every query of the Poisson trace has
+    the plain engine's reached count and depth, the dense sample gives the
+    plain engine's full level arrays, and one query per graph the oracle's."""
+    gs, (gi, so, arr), (g2, s2, arr2), plain, plain_lv = config5_setup
     with dfq.Server(gs, xcds_per_team=1) as srv:
         srv.start()
         say("server", srv.info())
@@ -232,6 +240,23 @@ def test_serve_config5_resident_graphs(dfq, oracle):
             np.testing.assert_array_equal(outs[q], oracle.bfs(g.row_offsets, g.col_idx, int(s2[q])),
                                           err_msg=f"oracle: query {q} graph {g2[q]}")
     say("graphs checked against the oracle:", sorted(seen))
+
+
+def test_serve_config5_group_tasks(dfq, config5_setup):
+    """Config 5 through group tasks on one-XCD teams (the served form
+    bench.py measures): the same trace and sample give the plain engine's
+    results, the sample's full level arrays included."""
+    gs, (gi, so, arr), (g2, s2, arr2), plain, plain_lv = config5_setup
+    with dfq.Server(gs, xcds_per_team=1, groups=True) as srv:
+        srv.start()
+        say("group server", srv.info())
+        recs, _ = srv.replay(gi, so, arr)
+        np.testing.assert_array_equal(recs["reached"], plain[:, 0])
+        np.testing.assert_array_equal(recs["depth"].astype(np.uint64), plain[:, 1])
+        _, outs = srv.replay(g2, s2, arr2, keep_levels=True)
+        srv.stop()
+    for q in range(len(g2)):
+        np.testing.assert_array_equal(outs[q], plain_lv[q], err_msg=f"group tasks: query {q} graph {g2[q]}")
 
 
 # ---- group mode: queries of one graph share traversals --------------------

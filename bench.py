@@ -274,7 +274,11 @@ def run_levels(args, dfq):
     level of the first --level-sources searches of the headline graph in
     --mode, each next to the plain launch's time, and the 4096^2 grid's
     levels summarised by frontier size (the profiling kernel adds per-level
-    atomics and timestamps, so its times run above the plain launch's)."""
+    atomics and timestamps, so its times run above the plain launch's).
+    For the grid, each level's time splits into the slowest workgroup's
+    busy time (its share of the level up to its barrier arrival) and the
+    rest: the barrier from the last arrival to the release workgroup 0
+    sees, and the start of the next level."""
     out = {"mode": args.mode, "rmat": [], "grid": {}}
     csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)
     srcs = csr.pick_sources(args.sources, seed=2022)
@@ -297,13 +301,17 @@ def run_levels(args, dfq):
         rows, st = g.profile(0, max_levels=8192)
         fr = np.array([r["frontier"] for r in rows])
         ns = np.array([r["level_ns"] for r in rows], dtype=np.float64)
+        busy = np.array([r["busy_max_ns"] for r in rows], dtype=np.float64)
         out["grid"] = {"plain_ms": round(plain.kernel_ms, 3), "profiled_ms": round(st.kernel_ms, 3),
                        "levels": len(rows), "by_frontier": []}
         for lo, hi in [(0, 64), (64, 256), (256, 1024), (1024, 2048), (2048, 4096), (4096, 1 << 30)]:
             m = (fr >= lo) & (fr < hi)
             if m.any():
                 out["grid"]["by_frontier"].append({"frontier": [lo, hi], "levels": int(m.sum()),
-                                                   "profiled_us_mean": round(float(ns[m].mean()) / 1e3, 2)})
+                                                   "profiled_us_mean": round(float(ns[m].mean()) / 1e3, 2),
+                                                   "busy_max_us_mean": round(float(busy[m].mean()) / 1e3, 2),
+                                                   "barrier_us_mean": round(float((ns[m] - busy[m]).mean()) / 1e3,
+                                                                            2)})
     return out
 
 

@@ -278,8 +278,10 @@ def run_levels(args, dfq):
     For the grid, each level's time splits into the slowest workgroup's
     busy time (its share of the level up to its barrier arrival) and the
     rest: the barrier from the last arrival to the release workgroup 0
-    sees, and the start of the next level."""
-    out = {"mode": args.mode, "rmat": [], "grid": {}}
+    sees, and the start of the next level.  "rmat_multi": the levels of
+    one multi-source traversal of the --sources sources (at most 64), the
+    search that group tasks of the served form run."""
+    out = {"mode": args.mode, "rmat": [], "rmat_multi": {}, "grid": {}}
     csr = dfq.Csr.rmat(args.scale, args.edgefactor, 0.57, 0.19, 0.19, seed=22)
     srcs = csr.pick_sources(args.sources, seed=2022)
     keep = ("frontier", "edges", "probes", "atomics", "claims", "items", "bottom_up", "level_ns",
@@ -293,6 +295,12 @@ def run_levels(args, dfq):
             out["rmat"].append({"source": int(s), "plain_ms": round(plain.kernel_ms, 4),
                                 "profiled_ms": round(st.kernel_ms, 4),
                                 "levels": [{k: r[k] for k in keep} for r in rows]})
+        ms_src = srcs[:64]
+        with dfq.DeviceBuffer(4 * csr.n * len(ms_src)) as d:
+            g.bfs_device_multi(ms_src, d.ptr)
+            _, ms = g.bfs_device_multi(ms_src, d.ptr)
+            rows, _ = g.multi_profile(ms_src, d.ptr)
+        out["rmat_multi"] = {"sources": len(ms_src), "traversal_ms": round(ms, 3), "levels": rows}
     c3 = dfq.Csr.grid2d(4096, 4096, 0.02, 4096)
     with dfq.DeviceGraph(c3) as g:
         g.set_config(direction_opt=0)

@@ -751,8 +751,10 @@ __device__ __forceinline__ void fresh_bit(uint32_t &word, bool need) {
 }
 
 // A top-down level takes a visited-bitmap snapshot when its frontier has at
-// least N / kSnapshotDiv out-edges.
+// least N / kSnapshotDiv out-edges and at least N / kSnapshotMinVisited
+// vertices are visited.
 constexpr uint64_t kSnapshotDiv = 16;
+constexpr uint64_t kSnapshotMinVisited = 64;
 
 // A top-down level without a snapshot whose frontier has at most this many
 // out-edges claims by the atomic alone (Visitor::direct): the probe before
@@ -1030,11 +1032,14 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       // visited bitmap taken at its start (read-only while the atomics hit
       // the live words); words copied while claims go on are still subsets
       // of the visited set, which is all the filter needs.  Smaller levels
-      // (and solo ones) probe the live bitmap alone.
-      const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n;
+      // (and solo ones) probe the live bitmap alone, and so does a large
+      // level while fewer than N / kSnapshotMinVisited vertices are
+      // visited: its snapshot would filter nothing.
+      const bool big = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n;
+      const bool snapshot = big && reached * kSnapshotMinVisited >= (uint64_t)a.n;
       vis.snap = snapshot ? a.vis_snap : a.vis;
       vis.probe_live = snapshot;
-      vis.direct = !snapshot && m_f <= kDirectMaxEdges;
+      vis.direct = !big && m_f <= kDirectMaxEdges;
       if (snapshot) {
         fresh_bit(fb_zero, true);
         for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];

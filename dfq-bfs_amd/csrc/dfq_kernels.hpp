@@ -1035,11 +1035,11 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       // (and solo ones) probe the live bitmap alone, and so does a large
       // level while fewer than N / kSnapshotMinVisited vertices are
       // visited: its snapshot would filter nothing.
-      const bool big = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n;
-      const bool snapshot = big && reached * kSnapshotMinVisited >= (uint64_t)a.n;
+      const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n &&
+                            reached * kSnapshotMinVisited >= (uint64_t)a.n;
       vis.snap = snapshot ? a.vis_snap : a.vis;
       vis.probe_live = snapshot;
-      vis.direct = !big && m_f <= kDirectMaxEdges;
+      vis.direct = !snapshot && m_f <= kDirectMaxEdges;
       if (snapshot) {
         fresh_bit(fb_zero, true);
         for (uint32_t w = gtid; w < a.n_words; w += gstride) a.vis_snap[w] = a.vis[w];

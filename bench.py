@@ -201,8 +201,8 @@ def batch_on(args, dfq, b, graphs, gi, so):
     b.set_queries(gi, so)
     t0 = time.perf_counter()
     tune = b.autotune(repeats=1)
-    log(f"autotune {time.perf_counter() - t0:.1f}s: block {tune['block_size']}, "
-        f"{tune['candidates']} candidates")
+    log(f"autotune {time.perf_counter() - t0:.1f}s: block {tune['block_size']}, grid-tier team xcds "
+        f"{tune['team_xcds']}, {tune['candidates']} candidates")
     for e in tune["buckets"]:
         log("  bucket", e["bucket"], "queries", e["queries"], e["plan"], f"{e['sub_batch_ms']:.3f} ms")
     for _ in range(args.warmup):
@@ -249,7 +249,8 @@ def batch_on(args, dfq, b, graphs, gi, so):
         "config": {"workload": f"{len(graphs)} undirected RMAT graphs, edgefactor 8, scales "
                                f"10..20 with p(n)~n^-1.5, {args.batch_sources} sources each",
                    "queries": nq, "vertices_total": int(sum(g.n for g in graphs)),
-                   "graph_replay": all(replays), "block_size": st.block_size},
+                   "graph_replay": all(replays), "block_size": st.block_size,
+                   "grid_tier_xcds_per_team": st.team_xcds},
         "batch_kernel_ms": round(kernel_ms, 3),
         # every query of a batch completes with its launch: a query's
         # latency is the launch time, the same for all of them

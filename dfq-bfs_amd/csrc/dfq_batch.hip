@@ -48,11 +48,14 @@ struct dfq_batch {
   dfq_query_plan plan[32];
   uint32_t block = 1024;
   int grid = 0;
-  // grid-tier scratch (largest graph) and workgroup-tier queues
+  // grid-tier scratch (largest graph), one copy per team, and
+  // workgroup-tier queues
+  uint32_t team_xcds = 0;   // XCDs per grid-tier team (0: the whole grid)
+  uint32_t teams_cap = 0;   // scratch copies allocated
   uint32_t *d_vis = nullptr, *d_vis_snap = nullptr, *d_fb = nullptr, *d_qseg[2] = {nullptr, nullptr};
   uint2 *d_items = nullptr;
   uint32_t item_cap = 0;
-  Control *d_ctl = nullptr;
+  Control *d_ctl = nullptr;  // one per team, then the census's
   uint32_t *d_work = nullptr;
   uint32_t *d_wg_queue = nullptr;
   int wg_queue_grid = 0;
@@ -97,6 +100,27 @@ void free_all(dfq_batch *b) {
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+// Grid-tier scratch for `copies` teams (each sized for the largest graph,
+// laid out as dfq_batch_kernel strides it) and copies + 1 control blocks.
+int alloc_team_scratch(dfq_batch *b, uint32_t copies) {
+  void *old[] = {b->d_vis, b->d_vis_snap, b->d_fb, b->d_qseg[0], b->d_qseg[1], b->d_items, b->d_ctl};
+  for (void *p : old) (void)hipFree(p);
+  b->d_vis = b->d_vis_snap = b->d_fb = b->d_qseg[0] = b->d_qseg[1] = nullptr;
+  b->d_items = nullptr;
+  b->d_ctl = nullptr;
+  b->teams_cap = 0;
+  const size_t nw = ((size_t)b->n_max + 31) / 32, c = copies;
+  BHIP(hipMalloc(&b->d_vis, sizeof(uint32_t) * nw * c));
+  BHIP(hipMalloc(&b->d_vis_snap, sizeof(uint32_t) * nw * c));
+  BHIP(hipMalloc(&b->d_fb, sizeof(uint32_t) * 2 * (nw + 1) * c));
+  for (int i = 0; i < 2; ++i) BHIP(hipMalloc(&b->d_qseg[i], sizeof(uint32_t) * (size_t)kSegments * b->n_max * c));
+  BHIP(hipMalloc(&b->d_items, sizeof(uint2) * (size_t)kSegments * b->item_cap * c));
+  BHIP(hipMalloc(&b->d_ctl, sizeof(Control) * (c + 1)));
+  BHIP(hipMemset(b->d_ctl, 0, sizeof(Control) * (c + 1)));
+  b->teams_cap = copies;
+  return DFQ_OK;
 }
 
 dfq_query_plan default_plan(uint32_t bucket) {
@@ -207,6 +231,9 @@ int build_tables(dfq_batch *b, const std::vector<uint32_t> &subset) {
   s.items = b->d_items;
   s.item_cap = b->item_cap;
   s.ctl = b->d_ctl;
+  a.xpt = b->team_xcds;
+  a.teams_cap = b->teams_cap;
+  a.n_max = b->n_max;
   s.prof = nullptr;
   s.prof_levels = 0;
   s.result = nullptr;
@@ -265,6 +292,8 @@ int launch(dfq_batch *b, bool use_graph, dfq_batch_stats *st) {
     st->graph_replay = replayed ? 1u : 0u;
     st->block_size = b->block;
     st->grid_blocks = (uint32_t)b->grid;
+    st->team_xcds = b->team_xcds;
+    st->reserved = 0;
   }
   return DFQ_OK;
 }
@@ -380,16 +409,11 @@ int dfq_batch_create(uint32_t num_graphs, const uint32_t *n, const uint64_t *m, 
     BFAIL(hipMemcpy(b->d_iso, iso.data(), sizeof(uint32_t) * iso.size(), hipMemcpyHostToDevice));
   }
   BFAIL(hipMemcpy(b->d_graphs, graphs.data(), sizeof(BatchGraph) * num_graphs, hipMemcpyHostToDevice));
-  // grid-tier scratch sized for the largest graph
-  const uint32_t nw = (b->n_max + 31) / 32;
-  BFAIL(hipMalloc(&b->d_vis, sizeof(uint32_t) * nw));
-  BFAIL(hipMalloc(&b->d_vis_snap, sizeof(uint32_t) * nw));
-  BFAIL(hipMalloc(&b->d_fb, sizeof(uint32_t) * 2 * ((size_t)nw + 1)));
-  for (int i = 0; i < 2; ++i) BFAIL(hipMalloc(&b->d_qseg[i], sizeof(uint32_t) * (size_t)kSegments * b->n_max));
+  // grid-tier scratch sized for the largest graph (one team: the grid)
   b->item_cap = (uint32_t)std::min<uint64_t>(b->m_max / kItem + b->n_max, 0xffffffffull);
-  BFAIL(hipMalloc(&b->d_items, sizeof(uint2) * (size_t)kSegments * b->item_cap));
-  BFAIL(hipMalloc(&b->d_ctl, sizeof(Control)));
-  BFAIL(hipMemset(b->d_ctl, 0, sizeof(Control)));
+  if (alloc_team_scratch(b, 1) != DFQ_OK) return fail(DFQ_ERR_HIP);
+  // the workgroup tier's instance counter (any start value: each launch
+  // counts from the value it finds)
   BFAIL(hipMalloc(&b->d_work, sizeof(uint32_t)));
 #undef BFAIL
   int rc = ensure_shape(b);
@@ -456,6 +480,37 @@ int dfq_batch_set_plan(dfq_batch *b, uint32_t bucket, const dfq_query_plan *plan
 int dfq_batch_get_plan(const dfq_batch *b, uint32_t bucket, dfq_query_plan *plan) {
   if (!b || !plan || bucket >= 32) return DFQ_ERR_INVALID_ARG;
   *plan = b->plan[bucket];
+  return DFQ_OK;
+}
+
+int dfq_batch_set_team_xcds(dfq_batch *b, uint32_t xcds_per_team) {
+  if (!b || (xcds_per_team != 0 && xcds_per_team != 1 && xcds_per_team != 2 && xcds_per_team != 4 &&
+             xcds_per_team != 8))
+    return DFQ_ERR_INVALID_ARG;
+  BHIP(hipSetDevice(b->device));
+  // teams the launch can form: the device's XCDs (the census's bound when
+  // the runtime does not say) over the team width
+  int xcc = 0;
+  if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, b->device) != hipSuccess || xcc <= 0 ||
+      xcc > kMaxXcd) {
+    (void)hipGetLastError();
+    xcc = kMaxXcd;
+  }
+  const uint32_t copies = xcds_per_team ? ((uint32_t)xcc + xcds_per_team - 1) / xcds_per_team : 1u;
+  if (copies != b->teams_cap) {
+    const int rc = alloc_team_scratch(b, copies);
+    if (rc != DFQ_OK) {
+      // keep a usable batch: the whole-grid layout
+      b->team_xcds = 0;
+      if (alloc_team_scratch(b, 1) != DFQ_OK) return DFQ_ERR_HIP;
+      drop_graph_exec(b);
+      b->tables_ok = false;
+      return DFQ_ERR_NOMEM;
+    }
+  }
+  b->team_xcds = xcds_per_team;
+  drop_graph_exec(b);
+  b->tables_ok = false;
   return DFQ_OK;
 }
 
@@ -620,6 +675,29 @@ int dfq_batch_autotune(dfq_batch *b, uint32_t repeats, dfq_tune_report *report) 
   int rc = dfq_batch_set_block_size(b, best_block);
   if (rc != DFQ_OK) return rc;
   rep.block_size = best_block;
+  // team width of the grid tier on the whole batch (several grid-tier
+  // queries at once on narrower teams)
+  best_ms = 1e30;
+  uint32_t best_x = b->team_xcds;
+  for (uint32_t x : {0u, 4u, 2u, 1u}) {
+    rc = dfq_batch_set_team_xcds(b, x);
+    if (rc != DFQ_OK) continue;
+    double ms = 1e30;
+    for (uint32_t r = 0; r < repeats + 1; ++r) {
+      dfq_batch_stats st;
+      rc = run_subset(b, {}, &st);
+      if (rc != DFQ_OK) return rc;
+      if (r > 0) ms = std::min(ms, st.kernel_ms);
+    }
+    ++rep.candidates;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_x = x;
+    }
+  }
+  rc = dfq_batch_set_team_xcds(b, best_x);
+  if (rc != DFQ_OK) return rc;
+  rep.team_xcds = best_x;
   drop_graph_exec(b);
   b->tables_ok = false;
   if (report) *report = rep;

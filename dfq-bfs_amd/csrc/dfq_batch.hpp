@@ -5,8 +5,11 @@
 // dfq_kernels.hpp).
 //
 // Two tiers, chosen per query by the host plan:
-//   grid tier       the whole grid runs one instance at a time (grid_bfs,
-//                   XCD-aware barrier per level) - for large graphs;
+//   grid tier       a team of XCDs runs one instance at a time (grid_bfs,
+//                   XCD-aware barrier per level) - for large graphs; with
+//                   teams narrower than the grid (BatchArgs::xpt) several
+//                   instances run at once, each team taking every
+//                   teams-th one;
 //   workgroup tier  one workgroup runs one instance: visited set and the
 //                   current / next frontier bitmaps live in LDS, claims are
 //                   LDS atomics, levels are separated by __syncthreads and the
@@ -65,7 +68,14 @@ struct BatchArgs {
   uint32_t *work;             // workgroup-tier instance counter
   uint32_t *wg_queue;         // per-workgroup frontier partitions [grid][2][kWgMaxN]
   const uint32_t *iso;        // concatenated isolated-vertex bitmaps (symmetric graphs)
-  BfsArgs grid;               // scratch of the grid tier (sized for its largest graph)
+  // scratch of the grid tier (sized for its largest graph), team 0's; team
+  // t's arrays follow at t times the strides below, and its control block
+  // is grid.ctl[t]; with teams, grid.ctl[teams_cap] is the census's
+  BfsArgs grid;
+  uint32_t xpt;               // XCDs per grid-tier team (0: the whole grid, one team)
+  uint32_t teams_cap;         // scratch copies (teams the launch may form)
+  uint32_t n_max;             // vertices of the largest graph (scratch strides)
+  uint32_t pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -383,19 +393,44 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
   __shared__ SharedBatch<BLOCK> sh;
   Team team{};
-  GridBarrier *bar = &b.grid.ctl->bar;
-  if (!kernel_prologue<BLOCK>(bar, 0, false, [bar](uint32_t) { return bar; }, sh.grid, team)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) b.grid.ctl->err = 2;
+  Control *ctl = b.grid.ctl;
+  const bool teams = b.xpt != 0;
+  GridBarrier *census = teams ? &ctl[b.teams_cap].bar : &ctl[0].bar;
+  // the workgroup tier's counter only grows: each launch counts from the
+  // value read here, before this workgroup is counted by the census, which
+  // no workgroup passes (and so takes an instance) before every one is
+  // counted
+  uint32_t work_base = 0;
+  if (threadIdx.x == 0) work_base = ld_relaxed(b.work);
+  if (!kernel_prologue<BLOCK>(census, b.xpt, teams, [ctl](uint32_t id) { return &ctl[id].bar; }, sh.grid,
+                              team)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[0].err = 2;
     return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *b.work = 0;
+  const uint32_t id = uniform(sh.grid.team_id);
+  if (threadIdx.x == 0) sh.grid.bcast[0] = team.teams;
+  __syncthreads();
+  const uint32_t nteams = uniform((uint32_t)sh.grid.bcast[0]);
+  __syncthreads();
+  GridBarrier *bar = &ctl[id].bar;
+  // every member has its rank once this barrier opens (teams rank their
+  // workgroups by ticket)
   grid_barrier(bar, team.bst, 0, sh.grid.bcast);
+  if (teams && sh.grid.team_rank == 0 && threadIdx.x == 0) team_ticket_reset(bar);
 
-  // grid tier: one instance at a time over the whole grid
-  for (uint32_t qi = 0; qi < b.n_grid; ++qi) {
+  // grid tier: this team's instances, one at a time over the team
+  const size_t nw = ((size_t)b.n_max + 31) / 32;
+  for (uint32_t qi = id; qi < b.n_grid; qi += nteams) {
     const BatchQuery q = b.queries[qi];
     const BatchGraph gr = b.graphs[q.graph];
     BfsArgs a = b.grid;
+    a.vis += id * nw;
+    a.vis_snap += id * nw;
+    a.fb += id * 2 * (nw + 1);
+    a.qseg[0] += id * (size_t)kSegments * b.n_max;
+    a.qseg[1] += id * (size_t)kSegments * b.n_max;
+    a.items += id * (size_t)kSegments * a.item_cap;
+    a.ctl = ctl + id;
     a.off = b.off + gr.vbase;
     a.col = b.col + gr.ebase;
     a.col_bu = gr.symmetric ? b.col_bu + gr.ebase : nullptr;
@@ -414,10 +449,10 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     a.alpha = q.alpha;
     a.bu_exit = q.beta ? gr.n / q.beta : 0;
     grid_bfs<BLOCK>(a, SearchIo{a.level, a.parent, a.result, a.source}, sh.grid, team);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      b.results[qi].depth = b.grid.ctl->depth;
-      b.results[qi].reached = b.grid.ctl->reached;
-      b.results[qi].bu_levels = b.grid.ctl->bu_levels;
+    if (sh.grid.team_rank == 0 && threadIdx.x == 0) {
+      b.results[qi].depth = a.ctl->depth;
+      b.results[qi].reached = a.ctl->reached;
+      b.results[qi].bu_levels = a.ctl->bu_levels;
     }
   }
   // the grid tier's level barriers leave caches as they are (grid_bfs
@@ -429,7 +464,8 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0)
-      sh.wg.query = b.n_grid + __hip_atomic_fetch_add(b.work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sh.wg.query = b.n_grid + (__hip_atomic_fetch_add(b.work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                                work_base);
     __syncthreads();
     const uint32_t qi = sh.wg.query;
     if (qi >= b.n_queries) break;

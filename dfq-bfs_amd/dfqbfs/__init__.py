@@ -551,7 +551,7 @@ class QueryPlan(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("queries", C.c_uint32), ("grid_queries", C.c_uint32),
                 ("wg_queries", C.c_uint32), ("graph_replay", C.c_uint32), ("block_size", C.c_uint32),
-                ("grid_blocks", C.c_uint32)]
+                ("grid_blocks", C.c_uint32), ("team_xcds", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class TuneEntry(C.Structure):
@@ -561,7 +561,7 @@ class TuneEntry(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("entries", C.c_uint32), ("block_size", C.c_uint32), ("candidates", C.c_uint32),
-                ("reserved", C.c_uint32), ("entry", TuneEntry * 32)]
+                ("team_xcds", C.c_uint32), ("entry", TuneEntry * 32)]
 
 
 _sig("dfq_batch_create", C.c_int, C.c_uint32, _u32p, C.POINTER(C.c_uint64), _i64p, _u32p,
@@ -571,6 +571,7 @@ _sig("dfq_batch_set_queries", C.c_int, _P, C.c_uint32, _u32p, _u32p)
 _sig("dfq_batch_set_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
 _sig("dfq_batch_get_plan", C.c_int, _P, C.c_uint32, C.POINTER(QueryPlan))
 _sig("dfq_batch_set_block_size", C.c_int, _P, C.c_uint32)
+_sig("dfq_batch_set_team_xcds", C.c_int, _P, C.c_uint32)
 _sig("dfq_batch_run", C.c_int, _P, C.POINTER(BatchStats))
 _sig("dfq_batch_levels", C.c_int, _P, C.c_uint32, _u32p)
 _sig("dfq_batch_result", C.c_int, _P, C.c_uint32, _u32p, C.POINTER(C.c_uint64))
@@ -629,6 +630,10 @@ class Batch:
     def set_block_size(self, block: int):
         check(lib.dfq_batch_set_block_size(self._h, block), "dfq_batch_set_block_size")
 
+    def set_team_xcds(self, xcds_per_team: int):
+        """Grid-tier queries on teams of this many XCDs, several at once (0: the whole grid)."""
+        check(lib.dfq_batch_set_team_xcds(self._h, xcds_per_team), "dfq_batch_set_team_xcds")
+
     def run(self) -> BatchStats:
         st = BatchStats()
         check(lib.dfq_batch_run(self._h, C.byref(st)), "dfq_batch_run")
@@ -647,7 +652,7 @@ class Batch:
     def autotune(self, repeats: int = 1):
         rep = TuneReport()
         check(lib.dfq_batch_autotune(self._h, repeats, C.byref(rep)), "dfq_batch_autotune")
-        return {"block_size": rep.block_size, "candidates": rep.candidates,
+        return {"block_size": rep.block_size, "candidates": rep.candidates, "team_xcds": rep.team_xcds,
                 "buckets": [dict(bucket=e.bucket, queries=e.queries, plan=e.plan.as_dict(),
                                  cost_us=e.cost_us, sub_batch_ms=e.sub_batch_ms)
                             for e in rep.entry[:rep.entries]]}

@@ -7,8 +7,9 @@
  * interactive loop :202-214,302-308 re-runs a search on the same resident
  * graph, one launch each).  Here a batch of (graph, source) queries runs in
  * one cooperative launch, replayed from a captured hipGraph: large graphs are
- * searched by the whole grid one after another, small ones by one workgroup
- * each with LDS-resident visited and frontier bitmaps.  Levels are the same
+ * searched by the whole grid (or by teams of XCDs, several at once) one
+ * after another, small ones by one workgroup each with LDS-resident visited
+ * and frontier bitmaps.  Levels are the same
  * as dfq_bfs's for every query.
  *
  * Graphs are given concatenated: graph g has n[g] vertices with local ids
@@ -56,6 +57,8 @@ typedef struct dfq_batch_stats {
   uint32_t graph_replay;   /* 1 when launched from a captured hipGraph     */
   uint32_t block_size;
   uint32_t grid_blocks;
+  uint32_t team_xcds;      /* XCDs per grid-tier team (0: the whole grid)  */
+  uint32_t reserved;
 } dfq_batch_stats;
 
 typedef struct dfq_tune_entry {
@@ -70,7 +73,7 @@ typedef struct dfq_tune_report {
   uint32_t entries;        /* filled entries of `entry`                    */
   uint32_t block_size;     /* chosen workgroup size                         */
   uint32_t candidates;     /* plans timed                                   */
-  uint32_t reserved;
+  uint32_t team_xcds;      /* chosen grid-tier team width (0: the whole grid) */
   dfq_tune_entry entry[32];
 } dfq_tune_report;
 
@@ -94,6 +97,11 @@ int dfq_batch_set_plan(dfq_batch *b, uint32_t bucket, const dfq_query_plan *plan
 int dfq_batch_get_plan(const dfq_batch *b, uint32_t bucket, dfq_query_plan *plan);
 /* 512 or 1024 threads per workgroup for the batched launch. */
 int dfq_batch_set_block_size(dfq_batch *b, uint32_t block_size);
+/* Grid-tier queries run on teams of this many XCDs (1, 2, 4 or 8), several
+ * at once, each team on its own working set (one copy per team); 0, the
+ * default, runs them on the whole grid one after another.  No counterpart
+ * in the reference (one search per process there). */
+int dfq_batch_set_team_xcds(dfq_batch *b, uint32_t xcds_per_team);
 
 /* Runs every query in one launch; blocks until done. */
 int dfq_batch_run(dfq_batch *b, dfq_batch_stats *stats);
@@ -107,8 +115,8 @@ int dfq_batch_result(const dfq_batch *b, uint32_t query, uint32_t *depth,
  * bucket's queries alone under candidate plans (tier x bin cut-offs x
  * direction optimisation), keeps the cheapest in workgroup-time per query,
  * moves buckets whose single query would outlast a quarter of the batch to
- * the grid tier, then picks the workgroup size on the whole batch.  The
- * chosen plans are set on the batch. */
+ * the grid tier, then picks the workgroup size and the grid tier's team
+ * width on the whole batch.  The choices are set on the batch. */
 int dfq_batch_autotune(dfq_batch *b, uint32_t repeats, dfq_tune_report *report);
 
 #ifdef __cplusplus

@@ -48,7 +48,8 @@ def check_all(b, gi, so, expect):
         assert depth == int(e[e != UNREACHED].max()) + 1
 
 
-@pytest.mark.parametrize("mode", ["default", "all_wg", "all_grid", "no_direction", "block512"])
+@pytest.mark.parametrize("mode", ["default", "all_wg", "all_grid", "no_direction", "block512", "teams1",
+                                  "all_grid_teams2"])
 def test_batch_levels_match_oracle(dfq, batch_setup, mode):
     gs, gi, so, expect = batch_setup
     with dfq.Batch(gs) as b:
@@ -56,19 +57,24 @@ def test_batch_levels_match_oracle(dfq, batch_setup, mode):
         for k in range(32):
             if mode == "all_wg" and (1 << k) <= dfq.WG_MAX_VERTICES:
                 b.set_plan(k, tier=dfq.TIER_WORKGROUP)
-            if mode == "all_grid":
+            if mode in ("all_grid", "all_grid_teams2"):
                 b.set_plan(k, tier=dfq.TIER_GRID)
             if mode == "no_direction":
                 b.set_plan(k, direction_opt=0)
         if mode == "block512":
             b.set_block_size(512)
+        if mode == "teams1":
+            b.set_team_xcds(1)  # grid-tier queries on one-XCD teams, several at once
+        if mode == "all_grid_teams2":
+            b.set_team_xcds(2)
         for rep in range(3):  # the first run captures the graph, the others replay it
             say("batch", mode, "run", rep)
             st = b.run()
             assert st.queries == len(gi)
             check_all(b, gi, so, expect)
-        if mode == "all_grid":
+        if mode in ("all_grid", "all_grid_teams2"):
             assert st.wg_queries == 0
+        assert st.team_xcds == {"teams1": 1, "all_grid_teams2": 2}.get(mode, 0)
         if mode == "all_wg":
             assert st.grid_queries == 3  # only the scale-18 graph
 
@@ -89,7 +95,7 @@ def test_batch_autotune_keeps_results(dfq, batch_setup):
         b.set_queries(gi, so)
         say("autotune")
         rep = b.autotune(repeats=1)
-        assert rep["candidates"] > 0 and rep["block_size"] in (512, 1024)
+        assert rep["candidates"] > 0 and rep["block_size"] in (512, 1024) and rep["team_xcds"] in (0, 1, 2, 4)
         buckets = {e["bucket"] for e in rep["buckets"]}
         assert 18 in buckets and rep["buckets"][-1]["plan"]["tier"] == dfq.TIER_GRID
         b.run()
@@ -132,7 +138,7 @@ def test_batch_config4_full_workload(dfq, oracle):
     with dfq.Batch(gs) as b:
         b.set_queries(gi, so)
         tune = b.autotune(repeats=1)
-        say("autotune: block", tune["block_size"], "buckets", len(tune["buckets"]))
+        say("autotune: block", tune["block_size"], "team xcds", tune["team_xcds"], "buckets", len(tune["buckets"]))
         runs = [b.run() for _ in range(3)]
         assert all(st.queries == 2048 for st in runs)
         assert [bool(st.graph_replay) for st in runs[1:]] == [True, True], \

@@ -556,19 +556,28 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
 }
 
 // Phase H: the level's work items, round-robin over every wavefront of the
-// grid.  `total` items located through sh.item_pre.
+// grid.  `total` items located through sh.item_pre.  A wave's items form a
+// software pipeline: while item i's columns, probes and claims are in
+// flight, item i+1's row offsets and item i+2's descriptor are loaded, so
+// an item costs the trips of its own edges only.
 template <int BLOCK, class V>
 __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis, uint32_t total,
                                 uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
 
   const uint32_t lane = lane_id();
-  for (uint32_t i = gwave; i < total; i += nwaves) {
+  auto item_at = [&](uint32_t i) {
     uint32_t j = 0;
 #pragma unroll
     for (int k = 1; k < kSegments; ++k) j += sh.item_pre[k] <= i;
-    const uint2 it = a.items[(size_t)j * a.item_cap + (i - sh.item_pre[j])];
+    return a.items[(size_t)j * a.item_cap + (i - sh.item_pre[j])];
+  };
+  uint32_t i = gwave;
+  if (i >= total) return;
+  uint2 it = item_at(i);
+  int64_t s0 = a.off[it.x], e0 = a.off[it.x + 1];
+  uint2 it1 = i + nwaves < total ? item_at(i + nwaves) : make_uint2(0u, 0u);
+  for (; i < total; i += nwaves) {
     const uint32_t u = it.x;
-    const int64_t s0 = a.off[u], e0 = a.off[u + 1];
     const int64_t s = s0 + (int64_t)it.y * kItem;
     const uint32_t len = (uint32_t)min((int64_t)kItem, e0 - s);
     if (lane == 0) edges += len;
@@ -581,7 +590,18 @@ __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOC
       uu[k] = u;
       w[k] = act[k] ? a.col[s + e] : 0u;
     }
+    // wave-uniform: total < 2^32 - 2 * nwaves (8 segments of item_cap)
+    int64_t s1 = 0, e1 = 0;
+    if (i + nwaves < total) {
+      s1 = a.off[it1.x];
+      e1 = a.off[it1.x + 1];
+    }
+    const uint2 it2 = i + 2 * nwaves < total ? item_at(i + 2 * nwaves) : make_uint2(0u, 0u);
     vis.template visit_batch<kUnroll, false>(act, uu, w);
+    it = it1;
+    s0 = s1;
+    e0 = e1;
+    it1 = it2;
   }
 }
 
@@ -937,7 +957,6 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   vis.parent = io.parent;
   vis.vis = a.vis;
   vis.snap = a.vis;
-  vis.probe_live = false;
   vis.direct = false;
   vis.stage = sh.stage[wave];
   vis.out_n = 0;
@@ -1031,14 +1050,15 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       // A level with many out-edges filters through a snapshot of the
       // visited bitmap taken at its start (read-only while the atomics hit
       // the live words); words copied while claims go on are still subsets
-      // of the visited set, which is all the filter needs.  Smaller levels
-      // (and solo ones) probe the live bitmap alone, and so does a large
-      // level while fewer than N / kSnapshotMinVisited vertices are
-      // visited: its snapshot would filter nothing.
+      // of the visited set, which is all the filter needs.  An entry the
+      // snapshot shows unvisited goes to the atomic without a second probe
+      // of the live words.  Smaller levels (and solo ones) probe the live
+      // bitmap alone, and so does a large level while fewer than
+      // N / kSnapshotMinVisited vertices are visited: its snapshot would
+      // filter nothing.
       const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n &&
                             reached * kSnapshotMinVisited >= (uint64_t)a.n;
       vis.snap = snapshot ? a.vis_snap : a.vis;
-      vis.probe_live = snapshot;
       vis.direct = !snapshot && m_f <= kDirectMaxEdges;
       if (snapshot) {
         fresh_bit(fb_zero, true);

@@ -461,6 +461,7 @@ struct Visitor {
   uint32_t *parent;     // may be null
   uint32_t *vis;
   const uint32_t *snap; // visited bitmap as of the level's start (subset of vis), or vis itself
+  bool probe_live;      // snap is a snapshot: probe the live bitmap after it
   bool direct;          // claim by the atomic alone, no bitmap probe before it
   uint32_t *seg;        // output segment of this workgroup's XCD
   uint32_t *seg_tail;   // its tail
@@ -502,20 +503,18 @@ struct Visitor {
   // claims (atomic OR for the few that look unvisited) and appends follow,
   // so one wave keeps U probes in flight instead of one.
   //
-  // Filter: one probe of `snap`, then the atomic for the neighbours it
-  // shows unvisited.  On large levels `snap` is the level-start snapshot of
-  // the visited bitmap (never written during the level, so its lines stay
-  // in L1/L2); a second probe of the live words for the neighbours
-  // unvisited at level start costs a round trip on every batch and saves
-  // few atomics, so there is none.  On other levels `snap` is the live
-  // bitmap.  The probe can only under-report visited vertices, never
-  // over-report.
+  // Filter: a probe of `snap`, then (probe_live) a probe of the live
+  // bitmap for the neighbours unvisited in `snap`, then the atomic for the
+  // neighbours still unvisited.  On large levels `snap` is the level-start
+  // snapshot of the visited bitmap (never written during the level, so its
+  // lines stay in L1/L2), on other levels the live bitmap, probed once.
+  // The loads can only under-report visited vertices, never over-report.
   //
   // kEarlyDeg (with deg32): the low words of the row offsets of the
   // vertices tried (for the out-degree sum) are loaded with the claim
   // attempts, so one wait covers both; otherwise the offsets are loaded
   // for the claimed vertices only, after the attempts return.
-  template <int U, bool kEarlyDeg>
+  template <int U, bool kEarlyDeg, bool kLive = false>
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
                                               const uint32_t (&w)[U]) {
     uint32_t word[U];
@@ -525,6 +524,11 @@ struct Visitor {
     } else {
 #pragma unroll
       for (int i = 0; i < U; ++i) word[i] = act[i] ? snap[w[i] >> 5] : 0xffffffffu;
+    }
+    if (kLive && probe_live) {
+#pragma unroll
+      for (int i = 0; i < U; ++i)
+        if (!((word[i] >> (w[i] & 31u)) & 1u)) word[i] = vis[w[i] >> 5];
     }
     const bool early = kEarlyDeg && track_mf && deg32;
     uint32_t old[U], lo0[U], lo1[U];

@@ -121,20 +121,38 @@ static_assert(DFQ_MULTI_PROFILE_FIELDS == kMsProfFields, "header and kernel disa
 
 // prof: the instantiation with the per-level counters compiled in (used by
 // dfq_bfs_profile only; the search kernels carry none of that code).
-const void *kernel_for(uint32_t block, bool prof = false) {
+// td: the top-down-only instantiation, for searches that cannot go
+// bottom-up (directed graph or direction_opt 0).
+template <int B>
+const void *kernel_at(bool prof, bool td) {
+  if (prof) return td ? (const void *)&dfq_bfs_kernel<B, true, true> : (const void *)&dfq_bfs_kernel<B, true>;
+  return td ? (const void *)&dfq_bfs_kernel<B, false, true> : (const void *)&dfq_bfs_kernel<B>;
+}
+
+const void *kernel_for(uint32_t block, bool prof = false, bool td = false) {
   switch (block) {
-    case 1024: return prof ? (const void *)&dfq_bfs_kernel<1024, true> : (const void *)&dfq_bfs_kernel<1024>;
-    case 512: return prof ? (const void *)&dfq_bfs_kernel<512, true> : (const void *)&dfq_bfs_kernel<512>;
-    default: return prof ? (const void *)&dfq_bfs_kernel<256, true> : (const void *)&dfq_bfs_kernel<256>;
+    case 1024: return kernel_at<1024>(prof, td);
+    case 512: return kernel_at<512>(prof, td);
+    default: return kernel_at<256>(prof, td);
   }
 }
 
-const void *seq_kernel_for(uint32_t block) {
+const void *seq_kernel_for(uint32_t block, bool td = false) {
   switch (block) {
-    case 1024: return (const void *)&dfq_bfs_seq_kernel<1024>;
-    case 512: return (const void *)&dfq_bfs_seq_kernel<512>;
-    default: return (const void *)&dfq_bfs_seq_kernel<256>;
+    case 1024: return td ? (const void *)&dfq_bfs_seq_kernel<1024, true> : (const void *)&dfq_bfs_seq_kernel<1024>;
+    case 512: return td ? (const void *)&dfq_bfs_seq_kernel<512, true> : (const void *)&dfq_bfs_seq_kernel<512>;
+    default: return td ? (const void *)&dfq_bfs_seq_kernel<256, true> : (const void *)&dfq_bfs_seq_kernel<256>;
   }
+}
+
+// Co-resident workgroups per CU of a kernel, the smaller of its two
+// instantiations (the grid is shared by both).
+hipError_t occupancy_both(int *per_cu, const void *k0, const void *k1, int block) {
+  int a = 0, b = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k0, block, 0);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k1, block, 0);
+  *per_cu = std::min(a, b);
+  return e;
 }
 
 // Grid = CUs x co-resident workgroups per CU (a cooperative launch needs
@@ -144,18 +162,18 @@ int configure(dfq_graph *g, const dfq_config &cfg) {
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   if (!prop.cooperativeLaunch) return DFQ_ERR_COOPERATIVE;
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(cfg.block_size),
-                                                        (int)cfg.block_size, 0));
+  HIP_TRY(occupancy_both(&per_cu, kernel_for(cfg.block_size), kernel_for(cfg.block_size, false, true),
+                         (int)cfg.block_size));
   if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
   if (cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)cfg.blocks_per_cu);
   int per_cu_seq = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_seq, seq_kernel_for(cfg.block_size),
-                                                        (int)cfg.block_size, 0));
+  HIP_TRY(occupancy_both(&per_cu_seq, seq_kernel_for(cfg.block_size), seq_kernel_for(cfg.block_size, true),
+                         (int)cfg.block_size));
   if (per_cu_seq <= 0) return DFQ_ERR_COOPERATIVE;
   per_cu_seq = std::min(per_cu_seq, per_cu);
   int per_cu_prof = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_prof, kernel_for(cfg.block_size, true),
-                                                        (int)cfg.block_size, 0));
+  HIP_TRY(occupancy_both(&per_cu_prof, kernel_for(cfg.block_size, true), kernel_for(cfg.block_size, true, true),
+                         (int)cfg.block_size));
   if (per_cu_prof <= 0) return DFQ_ERR_COOPERATIVE;
   per_cu_prof = std::min(per_cu_prof, per_cu);
   g->cfg = cfg;
@@ -280,7 +298,7 @@ void launch_kind(dfq_graph *g, BfsArgs &a, bool plain) {
 hipError_t launch_search(dfq_graph *g, BfsArgs &a) {
   void *args[] = {&a};
   const bool prof = a.prof != nullptr;
-  const void *k = kernel_for(g->cfg.block_size, prof);
+  const void *k = kernel_for(g->cfg.block_size, prof, !a.bottom_up);
   const int grid = prof ? g->grid_prof : g->grid;
   if (a.plain) return hipLaunchKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
   return hipLaunchCooperativeKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
@@ -588,8 +606,8 @@ int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources, ui
   const uint32_t *d_src = g->d_src;
   void *args[] = {&a, &d_src, &count, &level_stride, &parent_stride};
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  HIP_TRY(hipLaunchCooperativeKernel(seq_kernel_for(g->cfg.block_size), dim3(g->grid_seq), dim3(g->block), args,
-                                     0, g->stream));
+  HIP_TRY(hipLaunchCooperativeKernel(seq_kernel_for(g->cfg.block_size, !a.bottom_up), dim3(g->grid_seq),
+                                     dim3(g->block), args, 0, g->stream));
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipEventSynchronize(g->ev1));
   uint32_t err = 0;

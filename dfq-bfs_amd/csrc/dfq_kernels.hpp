@@ -560,7 +560,7 @@ __device__ __forceinline__ void topdown_small(const BfsArgs &a, SharedAll<BLOCK>
 // software pipeline: while item i's columns, probes and claims are in
 // flight, item i+1's row offsets and item i+2's descriptor are loaded, so
 // an item costs the trips of its own edges only.
-template <int BLOCK, class V>
+template <int BLOCK, bool kLive = false, class V>
 __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis, uint32_t total,
                                 uint32_t gwave, uint32_t nwaves, uint64_t &edges) {
 
@@ -597,7 +597,7 @@ __device__ __forceinline__ void topdown_phase_h(const BfsArgs &a, SharedAll<BLOC
       e1 = a.off[it1.x + 1];
     }
     const uint2 it2 = i + 2 * nwaves < total ? item_at(i + 2 * nwaves) : make_uint2(0u, 0u);
-    vis.template visit_batch<kUnroll, false>(act, uu, w);
+    vis.template visit_batch<kUnroll, false, kLive>(act, uu, w);
     it = it1;
     s0 = s1;
     e0 = e1;
@@ -776,6 +776,14 @@ __device__ __forceinline__ void fresh_bit(uint32_t &word, bool need) {
 constexpr uint64_t kSnapshotDiv = 16;
 constexpr uint64_t kSnapshotMinVisited = 64;
 
+// A snapshot level also probes the live bitmap (for the neighbours its
+// snapshot shows unvisited) when its frontier has more than
+// kLiveProbeRatio out-edges per visited vertex: most of its entries then
+// name vertices claimed during the level, and the probe saves their
+// atomics.  On a well-filled snapshot the probe is one more round trip
+// per batch that saves few atomics.
+constexpr uint64_t kLiveProbeRatio = 256;
+
 // A top-down level without a snapshot whose frontier has at most this many
 // out-edges claims by the atomic alone (Visitor::direct): the probe before
 // it is one more wait on a level that is all latency, and the extra atomics
@@ -925,7 +933,13 @@ struct GridBfsMode {
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
 // kProf: per-level counters into a.prof (compiled into the profiling kernel
 // only, so the search kernels carry none of their registers).
-template <int BLOCK, bool kProf = false>
+// kTd: every level top-down (a.bottom_up is ignored), compiled without the
+// bottom-up code; the work items of a snapshot level with more than
+// kLiveProbeRatio out-edges per visited vertex also probe the live bitmap.
+// That probe stays out of the kernels that may go bottom-up: the symmetric
+// graphs they serve reach snapshot levels in top-down mode only, and its
+// code alone cost the hybrid RMAT-22 search ~1 us.
+template <int BLOCK, bool kProf = false, bool kTd = false>
 __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, SharedAll<BLOCK> &sh, Team &team,
                                          GridBfsMode mode = GridBfsMode{false, false, 0}) {
   using S = KernelShape<BLOCK>;
@@ -957,6 +971,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   vis.parent = io.parent;
   vis.vis = a.vis;
   vis.snap = a.vis;
+  vis.probe_live = false;
   vis.direct = false;
   vis.stage = sh.stage[wave];
   vis.out_n = 0;
@@ -1007,7 +1022,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         ctl->slot[(L + 2) & 3].item_tail[k] = 0;
       }
     // direction choice (Beamer et al.): identical inputs in every workgroup
-    if (a.bottom_up) {
+    if (!kTd && a.bottom_up) {
       const uint64_t m_u = a.m > m_seen ? a.m - m_seen : 0;
       if (!bottom_up && n_cur > n_prev && m_f * a.alpha > m_u) bottom_up = true;
       else if (bottom_up && n_cur < a.bu_exit && n_cur < n_prev) bottom_up = false;
@@ -1050,15 +1065,14 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       // A level with many out-edges filters through a snapshot of the
       // visited bitmap taken at its start (read-only while the atomics hit
       // the live words); words copied while claims go on are still subsets
-      // of the visited set, which is all the filter needs.  An entry the
-      // snapshot shows unvisited goes to the atomic without a second probe
-      // of the live words.  Smaller levels (and solo ones) probe the live
-      // bitmap alone, and so does a large level while fewer than
-      // N / kSnapshotMinVisited vertices are visited: its snapshot would
-      // filter nothing.
+      // of the visited set, which is all the filter needs.  Smaller levels
+      // (and solo ones) probe the live bitmap alone, and so does a large
+      // level while fewer than N / kSnapshotMinVisited vertices are
+      // visited: its snapshot would filter nothing.
       const bool snapshot = !solo && m_f * kSnapshotDiv >= (uint64_t)a.n &&
                             reached * kSnapshotMinVisited >= (uint64_t)a.n;
       vis.snap = snapshot ? a.vis_snap : a.vis;
+      vis.probe_live = kTd && snapshot && m_f > reached * kLiveProbeRatio;
       vis.direct = !snapshot && m_f <= kDirectMaxEdges;
       if (snapshot) {
         fresh_bit(fb_zero, true);
@@ -1069,7 +1083,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       // one when it is known to be zero.  Solo levels and shrinking
       // frontiers leave the bitmaps alone (a bottom-up level after them
       // builds its bitmap from the level array).
-      if (a.bottom_up && !solo && n_cur > n_prev) {
+      if (!kTd && a.bottom_up && !solo && n_cur > n_prev) {
         uint32_t *fb_here = a.fb + (L & 1) * fbs;
         for (size_t w = gtid; w < fbs; w += gstride) fb_here[w] = 0;
         fb_track = (fb_zero >> ((L + 1) & 1)) & 1u;
@@ -1112,7 +1126,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         if (kProf && a.prof && tid == 0) th = now_ns();
         // a workgroup reads work items only when its first wave has one
         fresh_bit(fb_zero, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
-        topdown_phase_h<BLOCK>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
+        topdown_phase_h<BLOCK, kTd>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
         const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
         const uint64_t wg_mf2 = block_sum_mf<BLOCK>(sh, vis);
         if (kProf && a.prof && tid == 0) busy += now_ns() - th;
@@ -1231,7 +1245,7 @@ __device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_
   return sh.bcast[0] <= (uint64_t)kSegments;
 }
 
-template <int BLOCK, bool kProf = false>
+template <int BLOCK, bool kProf = false, bool kTd = false>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
@@ -1266,14 +1280,14 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   if (threadIdx.x == 0) l2_invalidate();
   __syncthreads();
   // the kernel ends after the search, which writes every L2 back
-  grid_bfs<BLOCK, kProf>(a, io, sh, team, GridBfsMode{true, true, solo_seen});
+  grid_bfs<BLOCK, kProf, kTd>(a, io, sh, team, GridBfsMode{true, true, solo_seen});
 }
 
 // Many searches of one graph in one launch, in turn: search i starts from
 // sources[i] and writes level + i * level_stride (and parent + i *
 // parent_stride when parents are asked for) and result + i * kResultWords.
 // Each search's first grid barrier also ends the previous one.
-template <int BLOCK>
+template <int BLOCK, bool kTd = false>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(const BfsArgs a, const uint32_t *sources, uint32_t count,
                                                             uint64_t level_stride, uint64_t parent_stride) {
   __shared__ SharedAll<BLOCK> sh;
@@ -1288,7 +1302,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(const BfsArgs a, con
                       a.result + (size_t)i * kResultWords, uniform(sources[i])};
     // the last search ends with the kernel; the others are followed by the
     // next one's initialisation, which needs their stores written back
-    grid_bfs<BLOCK>(a, io, sh, team, GridBfsMode{false, i + 1 == count, 0});
+    grid_bfs<BLOCK, false, kTd>(a, io, sh, team, GridBfsMode{false, i + 1 == count, 0});
   }
 }
 

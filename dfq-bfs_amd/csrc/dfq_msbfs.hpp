@@ -131,7 +131,8 @@ struct MsVisitor {
   // first skips the atomic when it already holds the bits (it only grows
   // during the level, so a stale copy can only hold fewer).  The atomic that
   // turns the mask from zero to nonzero appends w to the next queue.
-  template <int U, bool kEarlyDeg>
+  // kLive: no meaning here (the single-search visitor's live-bitmap probe).
+  template <int U, bool kEarlyDeg, bool kLive = false>
   __device__ __forceinline__ void visit_batch(const bool (&act)[U], const uint32_t (&u)[U],
                                               const uint32_t (&w)[U]) {
     uint64_t nw[U], cur[U];

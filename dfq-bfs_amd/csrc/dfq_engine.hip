@@ -57,6 +57,7 @@ struct dfq_graph {
   uint64_t m = 0;
   uint32_t n_words = 0;
   bool symmetric = false;
+  uint64_t max_deg = 0;  // largest out-degree
   int64_t *d_off = nullptr;
   uint32_t *d_col = nullptr;
   uint32_t *d_level = nullptr;   // outputs for the host-array entry point
@@ -122,37 +123,64 @@ static_assert(DFQ_MULTI_PROFILE_FIELDS == kMsProfFields, "header and kernel disa
 // prof: the instantiation with the per-level counters compiled in (used by
 // dfq_bfs_profile only; the search kernels carry none of that code).
 // td: the top-down-only instantiation, for searches that cannot go
-// bottom-up (directed graph or direction_opt 0).
+// bottom-up (directed graph or direction_opt 0).  low (with td, 1024-thread
+// blocks): the graph's largest out-degree fits the small bin.
+enum class Variant { kAny, kTd, kTdLow };
+
 template <int B>
-const void *kernel_at(bool prof, bool td) {
-  if (prof) return td ? (const void *)&dfq_bfs_kernel<B, true, true> : (const void *)&dfq_bfs_kernel<B, true>;
-  return td ? (const void *)&dfq_bfs_kernel<B, false, true> : (const void *)&dfq_bfs_kernel<B>;
+const void *kernel_at(bool prof, Variant v) {
+  if constexpr (B == 1024)
+    if (v == Variant::kTdLow)
+      return prof ? (const void *)&dfq_bfs_kernel<B, true, true, true>
+                  : (const void *)&dfq_bfs_kernel<B, false, true, true>;
+  if (v != Variant::kAny)
+    return prof ? (const void *)&dfq_bfs_kernel<B, true, true> : (const void *)&dfq_bfs_kernel<B, false, true>;
+  return prof ? (const void *)&dfq_bfs_kernel<B, true> : (const void *)&dfq_bfs_kernel<B>;
 }
 
-const void *kernel_for(uint32_t block, bool prof = false, bool td = false) {
+template <int B>
+const void *seq_kernel_at(Variant v) {
+  if constexpr (B == 1024)
+    if (v == Variant::kTdLow) return (const void *)&dfq_bfs_seq_kernel<B, true, true>;
+  if (v != Variant::kAny) return (const void *)&dfq_bfs_seq_kernel<B, true>;
+  return (const void *)&dfq_bfs_seq_kernel<B>;
+}
+
+const void *kernel_for(uint32_t block, bool prof = false, Variant v = Variant::kAny) {
   switch (block) {
-    case 1024: return kernel_at<1024>(prof, td);
-    case 512: return kernel_at<512>(prof, td);
-    default: return kernel_at<256>(prof, td);
+    case 1024: return kernel_at<1024>(prof, v);
+    case 512: return kernel_at<512>(prof, v);
+    default: return kernel_at<256>(prof, v);
   }
 }
 
-const void *seq_kernel_for(uint32_t block, bool td = false) {
+const void *seq_kernel_for(uint32_t block, Variant v = Variant::kAny) {
   switch (block) {
-    case 1024: return td ? (const void *)&dfq_bfs_seq_kernel<1024, true> : (const void *)&dfq_bfs_seq_kernel<1024>;
-    case 512: return td ? (const void *)&dfq_bfs_seq_kernel<512, true> : (const void *)&dfq_bfs_seq_kernel<512>;
-    default: return td ? (const void *)&dfq_bfs_seq_kernel<256, true> : (const void *)&dfq_bfs_seq_kernel<256>;
+    case 1024: return seq_kernel_at<1024>(v);
+    case 512: return seq_kernel_at<512>(v);
+    default: return seq_kernel_at<256>(v);
   }
 }
 
-// Co-resident workgroups per CU of a kernel, the smaller of its two
-// instantiations (the grid is shared by both).
-hipError_t occupancy_both(int *per_cu, const void *k0, const void *k1, int block) {
-  int a = 0, b = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k0, block, 0);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k1, block, 0);
-  *per_cu = std::min(a, b);
-  return e;
+// The instantiation a search with these arguments runs.
+Variant variant_for(const dfq_graph *g, const BfsArgs &a) {
+  if (a.bottom_up) return Variant::kAny;
+  return g->max_deg <= a.small_max ? Variant::kTdLow : Variant::kTd;
+}
+
+// Co-resident workgroups per CU of a kernel, the smallest over its
+// instantiations (the grid is shared by all).
+template <class F>
+hipError_t occupancy_all(int *per_cu, F kernel, int block) {
+  int best = 1 << 30;
+  for (Variant v : {Variant::kAny, Variant::kTd, Variant::kTdLow}) {
+    int k = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, kernel(v), block, 0);
+    if (e != hipSuccess) return e;
+    best = std::min(best, k);
+  }
+  *per_cu = best;
+  return hipSuccess;
 }
 
 // Grid = CUs x co-resident workgroups per CU (a cooperative launch needs
@@ -162,18 +190,16 @@ int configure(dfq_graph *g, const dfq_config &cfg) {
   HIP_TRY(hipGetDeviceProperties(&prop, g->device));
   if (!prop.cooperativeLaunch) return DFQ_ERR_COOPERATIVE;
   int per_cu = 0;
-  HIP_TRY(occupancy_both(&per_cu, kernel_for(cfg.block_size), kernel_for(cfg.block_size, false, true),
-                         (int)cfg.block_size));
+  const uint32_t bs = cfg.block_size;
+  HIP_TRY(occupancy_all(&per_cu, [bs](Variant v) { return kernel_for(bs, false, v); }, (int)bs));
   if (per_cu <= 0) return DFQ_ERR_COOPERATIVE;
   if (cfg.blocks_per_cu) per_cu = std::min(per_cu, (int)cfg.blocks_per_cu);
   int per_cu_seq = 0;
-  HIP_TRY(occupancy_both(&per_cu_seq, seq_kernel_for(cfg.block_size), seq_kernel_for(cfg.block_size, true),
-                         (int)cfg.block_size));
+  HIP_TRY(occupancy_all(&per_cu_seq, [bs](Variant v) { return seq_kernel_for(bs, v); }, (int)bs));
   if (per_cu_seq <= 0) return DFQ_ERR_COOPERATIVE;
   per_cu_seq = std::min(per_cu_seq, per_cu);
   int per_cu_prof = 0;
-  HIP_TRY(occupancy_both(&per_cu_prof, kernel_for(cfg.block_size, true), kernel_for(cfg.block_size, true, true),
-                         (int)cfg.block_size));
+  HIP_TRY(occupancy_all(&per_cu_prof, [bs](Variant v) { return kernel_for(bs, true, v); }, (int)bs));
   if (per_cu_prof <= 0) return DFQ_ERR_COOPERATIVE;
   per_cu_prof = std::min(per_cu_prof, per_cu);
   g->cfg = cfg;
@@ -298,7 +324,7 @@ void launch_kind(dfq_graph *g, BfsArgs &a, bool plain) {
 hipError_t launch_search(dfq_graph *g, BfsArgs &a) {
   void *args[] = {&a};
   const bool prof = a.prof != nullptr;
-  const void *k = kernel_for(g->cfg.block_size, prof, !a.bottom_up);
+  const void *k = kernel_for(g->cfg.block_size, prof, variant_for(g, a));
   const int grid = prof ? g->grid_prof : g->grid;
   if (a.plain) return hipLaunchKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
   return hipLaunchCooperativeKernel(k, dim3(grid), dim3(g->block), args, 0, g->stream);
@@ -392,12 +418,19 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   if (m > kMaxEntries) return DFQ_ERR_RANGE;
   if (row_offsets[0] != 0 || (uint64_t)row_offsets[n] != m) return DFQ_ERR_RANGE;
   std::atomic<bool> bad{false};
+  std::atomic<uint64_t> max_deg{0};
   parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
-    for (uint64_t v = b; v < e; ++v)
+    uint64_t mx = 0;
+    for (uint64_t v = b; v < e; ++v) {
       if (row_offsets[v + 1] < row_offsets[v]) {
         bad = true;
         return;
       }
+      mx = std::max(mx, (uint64_t)(row_offsets[v + 1] - row_offsets[v]));
+    }
+    uint64_t cur = max_deg.load();
+    while (mx > cur && !max_deg.compare_exchange_weak(cur, mx)) {
+    }
   });
   if (bad) return DFQ_ERR_RANGE;
   parallel_for(m, [&](uint64_t b, uint64_t e, unsigned) {
@@ -431,6 +464,7 @@ int dfq_graph_create(const int64_t *row_offsets, const uint32_t *col_idx, uint32
   g->m = m;
   g->n_words = (n + 31) / 32;
   g->symmetric = is_symmetric(row_offsets, col_idx, n);
+  g->max_deg = max_deg.load();
   HIP_OR_FAIL(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   HIP_OR_FAIL(hipEventCreate(&g->ev0));
   HIP_OR_FAIL(hipEventCreate(&g->ev1));
@@ -606,7 +640,7 @@ int dfq_bfs_device_seq(dfq_graph *g, uint32_t count, const uint32_t *sources, ui
   const uint32_t *d_src = g->d_src;
   void *args[] = {&a, &d_src, &count, &level_stride, &parent_stride};
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  HIP_TRY(hipLaunchCooperativeKernel(seq_kernel_for(g->cfg.block_size, !a.bottom_up), dim3(g->grid_seq),
+  HIP_TRY(hipLaunchCooperativeKernel(seq_kernel_for(g->cfg.block_size, variant_for(g, a)), dim3(g->grid_seq),
                                      dim3(g->block), args, 0, g->stream));
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipEventSynchronize(g->ev1));

@@ -382,10 +382,13 @@ __device__ __forceinline__ void prof_flush(const BfsArgs &a, SharedAll<BLOCK> &s
 
 // Top-down phase A over this workgroup's equal share of the frontier.
 // kProf: the per-level counters are compiled in (the profiling kernel only).
+// kLow: every frontier vertex is known to be in the small bin (the graph's
+// largest out-degree is at most a.small_max), so the other bins, their LDS
+// queues and the workgroup barriers around them are compiled out.
 // Shared data is first read after the workgroup barrier that follows the
 // segment tails, so an invalidate by thread 0 before the call is ordered
 // before those reads.
-template <int BLOCK, bool kProf, class V>
+template <int BLOCK, bool kProf, bool kLow = false, class V>
 __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOCK> &sh, V &vis,
                                 const uint32_t *in_seg, const LevelSlot &in_slot, LevelSlot &cur,
                                 uint32_t wave, uint32_t trank, uint32_t tsize, uint64_t &edges) {
@@ -418,12 +421,12 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
         v = in_seg[(size_t)j * a.n + (p - sh.seg_pre[j])];
         start = a.off[v];
         deg = (uint32_t)(a.off[v + 1] - start);
-        if (deg <= a.large_max) edges += deg;
+        if (kLow || deg <= a.large_max) edges += deg;
       }
-      const bool is_small = deg != 0 && deg <= a.small_max;
-      const bool is_med = deg > a.small_max && deg <= a.medium_max;
-      const bool is_big = deg > a.medium_max && deg <= a.large_max;
-      const bool is_huge = deg > a.large_max;
+      const bool is_small = deg != 0 && (kLow || deg <= a.small_max);
+      const bool is_med = !kLow && deg > a.small_max && deg <= a.medium_max;
+      const bool is_big = !kLow && deg > a.medium_max && deg <= a.large_max;
+      const bool is_huge = !kLow && deg > a.large_max;
       if (kProf && a.prof) prof_bins<BLOCK>(sh, is_small, is_med, is_big, is_huge, deg);
       // wave-aggregated slot reservation (ballot + mbcnt) for the bins
       const uint64_t mm = __ballot(is_med);
@@ -466,6 +469,7 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
       }
       gather_small<BLOCK>(vis, is_small ? v : 0u, is_small ? start : 0, is_small ? deg : 0u,
                           sh.u.a.g_excl[wave], sh.u.a.g_v[wave], sh.u.a.g_s[wave]);
+      if (kLow) continue;  // gather_small is wave-local: no LDS queue to drain
       __syncthreads();
       const uint32_t mn = sh.med_n, bn = sh.big_n;
       __syncthreads();
@@ -483,6 +487,7 @@ __device__ __forceinline__ void topdown_phase_a(const BfsArgs &a, SharedAll<BLOC
     }
   }
   __syncthreads();
+  if (kLow) return;
   // drain (reference :161-169)
   const uint32_t mn = sh.med_n, bn = sh.big_n;
   expand_medium<BLOCK>(vis, sh, mn, wave);
@@ -933,13 +938,16 @@ struct GridBfsMode {
 // vertex.  Results: the output arrays plus ctl->depth / reached / bu_levels.
 // kProf: per-level counters into a.prof (compiled into the profiling kernel
 // only, so the search kernels carry none of their registers).
+// kLow (with kTd): the graph's largest out-degree is at most a.small_max:
+// no medium, large or huge bins, no work items, no edge-balanced short
+// frontiers (see topdown_phase_a).
 // kTd: every level top-down (a.bottom_up is ignored), compiled without the
 // bottom-up code; the work items of a snapshot level with more than
 // kLiveProbeRatio out-edges per visited vertex also probe the live bitmap.
 // That probe stays out of the kernels that may go bottom-up: the symmetric
 // graphs they serve reach snapshot levels in top-down mode only, and its
 // code alone cost the hybrid RMAT-22 search ~1 us.
-template <int BLOCK, bool kProf = false, bool kTd = false>
+template <int BLOCK, bool kProf = false, bool kTd = false, bool kLow = false>
 __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, SharedAll<BLOCK> &sh, Team &team,
                                          GridBfsMode mode = GridBfsMode{false, false, 0}) {
   using S = KernelShape<BLOCK>;
@@ -1093,7 +1101,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       fb_valid = fb_track;
       // a short frontier with many edges (hubs): edge-balanced over the
       // team in one pass; otherwise degree bins
-      if (!solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull) {
+      if (!kLow && !solo && n_cur <= (uint64_t)BLOCK && m_f >= kSmallFrontierEdges && m_f <= 0xffffffffull) {
         fresh_bit(fb_zero, true);  // every workgroup reads the whole frontier
         topdown_small<BLOCK>(a, sh, vis, in_seg, in_slot, wave, gwave, nwaves, edges);
       } else {
@@ -1104,7 +1112,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
           if (tid == 0) l2_invalidate();
           fb_zero |= kFresh;
         }
-        topdown_phase_a<BLOCK, kProf>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
+        topdown_phase_a<BLOCK, kProf, kLow>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
       }
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
       const uint64_t wg_mf = block_sum_mf<BLOCK>(sh, vis);
@@ -1117,9 +1125,11 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         fb_zero &= ~kFresh;
       }
       m_f = mf1;
-      if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
-      __syncthreads();
-      hc = uniform(sh.item_pre[kSegments]);
+      if (!kLow) {
+        if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
+        __syncthreads();
+        hc = uniform(sh.item_pre[kSegments]);
+      }
       if (hc) {
         const uint32_t before = vis.claims;
         uint64_t th = 0;
@@ -1245,7 +1255,7 @@ __device__ __forceinline__ bool kernel_prologue(GridBarrier *census_bar, uint32_
   return sh.bcast[0] <= (uint64_t)kSegments;
 }
 
-template <int BLOCK, bool kProf = false, bool kTd = false>
+template <int BLOCK, bool kProf = false, bool kTd = false, bool kLow = false>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   __shared__ SharedAll<BLOCK> sh;
   Team team{};
@@ -1280,14 +1290,14 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_kernel(BfsArgs a) {
   if (threadIdx.x == 0) l2_invalidate();
   __syncthreads();
   // the kernel ends after the search, which writes every L2 back
-  grid_bfs<BLOCK, kProf, kTd>(a, io, sh, team, GridBfsMode{true, true, solo_seen});
+  grid_bfs<BLOCK, kProf, kTd, kLow>(a, io, sh, team, GridBfsMode{true, true, solo_seen});
 }
 
 // Many searches of one graph in one launch, in turn: search i starts from
 // sources[i] and writes level + i * level_stride (and parent + i *
 // parent_stride when parents are asked for) and result + i * kResultWords.
 // Each search's first grid barrier also ends the previous one.
-template <int BLOCK, bool kTd = false>
+template <int BLOCK, bool kTd = false, bool kLow = false>
 __global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(const BfsArgs a, const uint32_t *sources, uint32_t count,
                                                             uint64_t level_stride, uint64_t parent_stride) {
   __shared__ SharedAll<BLOCK> sh;
@@ -1302,7 +1312,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_bfs_seq_kernel(const BfsArgs a, con
                       a.result + (size_t)i * kResultWords, uniform(sources[i])};
     // the last search ends with the kernel; the others are followed by the
     // next one's initialisation, which needs their stores written back
-    grid_bfs<BLOCK, false, kTd>(a, io, sh, team, GridBfsMode{false, i + 1 == count, 0});
+    grid_bfs<BLOCK, false, kTd, kLow>(a, io, sh, team, GridBfsMode{false, i + 1 == count, 0});
   }
 }
 

@@ -266,6 +266,7 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.n_words = g->n_words;
   a.source = source;
   a.small_max = g->cfg.small_max;
+  a.deg_max = (uint32_t)std::min<uint64_t>(g->max_deg, 0xffffffffu);
   a.medium_max = g->cfg.medium_max;
   a.large_max = g->cfg.large_max;
   a.bottom_up = (g->cfg.direction_opt && g->symmetric) ? 1u : 0u;
@@ -295,6 +296,7 @@ void fill_stats(const dfq_graph *g, const uint64_t *r, double ms, int grid, dfq_
   st.kernel_ms = ms;
   st.depth = (uint32_t)r[0];
   st.reached = r[1];
+  st.edges = r[3];
   st.bottom_up_levels = (uint32_t)r[2];
   st.grid_blocks = (uint32_t)grid;
   st.block_size = (uint32_t)g->block;
@@ -359,6 +361,7 @@ int run_bfs(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_parent
     stats->kernel_ms = ms;
     stats->depth = c.depth;
     stats->reached = c.reached;
+    stats->edges = c.edges;
     stats->grid_blocks = (uint32_t)(prof ? g->grid_prof : g->grid);
     stats->block_size = (uint32_t)g->block;
     stats->bottom_up_levels = c.bu_levels;
@@ -819,6 +822,7 @@ int run_multi(dfq_graph *g, uint32_t count, const uint32_t *sources, uint32_t *d
     stats[i].kernel_ms = ms;
     stats[i].depth = dep[i];
     stats[i].reached = cnt[i];
+    stats[i].edges = cnt[kMsMaxSources + i];
     stats[i].grid_blocks = (uint32_t)g->grid_ms;
     stats[i].block_size = 1024;
     stats[i].bottom_up_levels = c.bu_levels;

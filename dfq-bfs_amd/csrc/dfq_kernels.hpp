@@ -62,7 +62,7 @@ struct Control {
   uint64_t reached;   // vertices with a level
   uint32_t bu_levels; // levels run bottom-up
   uint32_t abandoned; // number of the last plain launch whose grid did not assemble
-  uint64_t edges;     // CSR entries of reached vertices (when counted)
+  uint64_t edges;     // CSR entries of reached vertices (single searches)
   uint32_t pad[56];
   BarrierWord solo_rel;        // tag of the last hand-back of solo levels
   uint64_t solo_state[32];     // search state handed back (first 8 words used)
@@ -126,6 +126,7 @@ struct BfsArgs {
   uint32_t alpha;        // bottom-up when frontier edges > unexplored edges / alpha
   uint32_t bu_exit;      // back to top-down when a shrinking frontier has fewer vertices
   uint64_t m;            // CSR entries
+  uint32_t deg_max = 0;  // largest out-degree (read by the low-degree instantiation only)
 };
 
 // What changes from one search to the next on the same resident graph:
@@ -313,20 +314,28 @@ __device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, ui
   return sum;
 }
 
-// Workgroup sum of the per-lane out-degree sums of claimed vertices, in
-// every thread; resets the lanes' counters.  Per wave a DPP sum (no LDS
-// permutes), then the wave sums through LDS.  Ends with a __syncthreads.
-template <int BLOCK, class V>
-__device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, V &vis) {
+// Workgroup sum of a per-lane value (the workgroup's total below 2^40), in
+// every thread.  Per wave a DPP sum (no LDS permutes), then the wave sums
+// through LDS.  Ends with a __syncthreads.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_sum_u40(SharedAll<BLOCK> &sh, uint64_t v) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
-  const uint64_t w = wave_sum_u40(vis.mf);
-  vis.mf = 0;
+  const uint64_t w = wave_sum_u40(v);
   if (lane_id() == 0) sh.wave_mf[threadIdx.x / kWave] = w;
   __syncthreads();
   uint64_t t = 0;
   for (int i = 0; i < W; ++i) t += sh.wave_mf[i];
   __syncthreads();  // the slots are free again
   return uniform(t);
+}
+
+// Workgroup sum of the per-lane out-degree sums of claimed vertices, in
+// every thread; resets the lanes' counters.
+template <int BLOCK, class V>
+__device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, V &vis) {
+  const uint64_t t = block_sum_u40<BLOCK>(sh, vis.mf);
+  vis.mf = 0;
+  return t;
 }
 
 // Sum over all lanes of the workgroup of a per-lane value into sh.prof[f]
@@ -940,7 +949,13 @@ struct GridBfsMode {
 // only, so the search kernels carry none of their registers).
 // kLow (with kTd): the graph's largest out-degree is at most a.small_max:
 // no medium, large or huge bins, no work items, no edge-balanced short
-// frontiers (see topdown_phase_a).
+// frontiers (see topdown_phase_a).  Nor is the next frontier's out-degree
+// sum counted per level (no degree loads for claimed vertices, no side sum
+// in the level barrier): the level rules take the bound
+// n_next * a.deg_max instead, and the CSR entries of reached vertices are
+// summed once, at the end of the search, from the degrees each thread
+// expanded (into result[3], or ctl->edges for a search without result
+// words, which is then the launch's only search).
 // kTd: every level top-down (a.bottom_up is ignored), compiled without the
 // bottom-up code; the work items of a snapshot level with more than
 // kLiveProbeRatio out-edges per visited vertex also probe the live bitmap.
@@ -963,6 +978,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   if (kProf && tid < kProfFields) sh.prof[tid] = 0;
 
   if (io.result && gtid == 0) io.result[4] = now_ns();
+  // kLow: summed by atomics at the end of the search; zeroed by an
+  // agent-scope store, which leaves no cached copy of the word behind
+  uint64_t *const e_sum = io.result ? &io.result[3] : &ctl->edges;
+  if (kLow && gtid == 0) __hip_atomic_store(e_sum, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t solo_next = 0;  // tag of the next hand-back of solo levels (thread 0)
   if (mode.pre_init) {
     if (tid == 0) solo_next = (mode.solo_seen & ~kSoloDone) + 1;
@@ -990,7 +1009,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   vis.fb_mark = nullptr;
   // out-degree sums of the claimed vertices: the direction heuristic, the
   // choice of the small-frontier expansion, and ctl->edges
-  vis.track_mf = true;
+  vis.track_mf = !kLow;
   vis.deg32 = a.m <= 0xffffffffull;  // no row can have 2^32 entries
   vis.mf = 0;
 
@@ -999,6 +1018,7 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
   // placed in a frontier so far (m - m_seen = edges left to explore)
   uint64_t m_f = (uint64_t)(a.off[io.source + 1] - a.off[io.source]);
   uint64_t m_seen = m_f;
+  uint64_t e_total = 0;  // kLow: CSR entries this thread expanded, all levels
   uint32_t bu_levels = 0;
   bool bottom_up = false;
   // Frontier bitmaps alternate by level parity: fb[L & 1] is level L's.
@@ -1115,16 +1135,17 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         topdown_phase_a<BLOCK, kProf, kLow>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
       }
       const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
-      const uint64_t wg_mf = block_sum_mf<BLOCK>(sh, vis);
+      const uint64_t wg_mf = kLow ? 0 : block_sum_mf<BLOCK>(sh, vis);
       if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
       uint64_t mf1 = 0;
       if (solo) {
         n_next = solo_sync<BLOCK>(sh, wg_claims, wg_mf, &mf1);
       } else {
-        n_next = grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, wg_mf, &mf1);
+        n_next = kLow ? grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast)
+                      : grid_barrier<false, false>(&ctl->bar, bst, wg_claims, sh.bcast, wg_mf, &mf1);
         fb_zero &= ~kFresh;
       }
-      m_f = mf1;
+      m_f = kLow ? n_next * a.deg_max : mf1;
       if (!kLow) {
         if (tid < kWave) tails_prefix<kSegments>(cur.item_tail, sh.item_pre);
         __syncthreads();
@@ -1194,18 +1215,19 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       vis.probes = 0;
       vis.atomics = 0;
     }
+    if (kLow) e_total += edges;
     reached += n_next;
     if (n_next == 0) {
       if (gtid == 0) {
         ctl->depth = L + 1;
         ctl->reached = reached;
         ctl->bu_levels = bu_levels;
-        ctl->edges = m_seen;
+        if (!kLow) ctl->edges = m_seen;
         if (io.result) {
           io.result[0] = L + 1;
           io.result[1] = reached;
           io.result[2] = bu_levels;
-          io.result[3] = m_seen;
+          if (!kLow) io.result[3] = m_seen;
           io.result[5] = now_ns();
         }
       }
@@ -1214,6 +1236,10 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
     }
     n_prev = n_cur;
     n_cur = n_next;
+  }
+  if (kLow) {
+    const uint64_t wg_e = block_sum_u40<BLOCK>(sh, e_total);
+    if (tid == 0 && wg_e) __hip_atomic_fetch_add(e_sum, wg_e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

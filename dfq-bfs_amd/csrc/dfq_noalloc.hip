@@ -213,6 +213,7 @@ extern "C" int dfq_bfs_device_noalloc(const int64_t *d_row_offsets, const uint32
     stats->kernel_ms = ms;
     stats->depth = c.depth;
     stats->reached = c.reached;
+    stats->edges = c.edges;
     stats->grid_blocks = (uint32_t)s->grid;
     stats->block_size = kNaBlock;
     stats->bottom_up_levels = c.bu_levels;

@@ -107,7 +107,7 @@ class Config(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("depth", C.c_uint32), ("grid_blocks", C.c_uint32),
                 ("block_size", C.c_uint32), ("bottom_up_levels", C.c_uint32),
-                ("reached", C.c_uint64)]
+                ("reached", C.c_uint64), ("edges", C.c_uint64)]
 
 
 class DeviceInfo(C.Structure):
@@ -343,6 +343,7 @@ class BfsResult:
     grid_blocks: int
     block_size: int
     bottom_up_levels: int = 0
+    edges: int = 0        # CSR entries of reached vertices (E_r)
 
 
 class DeviceGraph:
@@ -397,7 +398,7 @@ class DeviceGraph:
         check(lib.dfq_bfs(self._h, source, _u32(level), _u32(parent) if parents else None,
                           C.byref(st)), "dfq_bfs")
         return BfsResult(level, parent, st.kernel_ms, st.depth, st.reached, st.grid_blocks,
-                         st.block_size, st.bottom_up_levels)
+                         st.block_size, st.bottom_up_levels, st.edges)
 
     @property
     def symmetric(self) -> bool:

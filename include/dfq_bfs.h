@@ -91,6 +91,7 @@ typedef struct dfq_stats {
   uint32_t block_size;       /* threads per workgroup                     */
   uint32_t bottom_up_levels; /* levels that ran bottom-up                 */
   uint64_t reached;          /* vertices with a level                     */
+  uint64_t edges;            /* CSR entries of reached vertices (E_r)     */
 } dfq_stats;
 
 typedef struct dfq_device_info {

@@ -20,6 +20,8 @@ def check_result(dfq, csr, source, res, expect):
     reached = int((expect != UNREACHED).sum())
     assert res.reached == reached
     assert res.depth == int(expect[expect != UNREACHED].max()) + 1
+    # E_r, the numerator of TEPS: CSR entries of the reached vertices
+    assert res.edges == int(np.diff(csr.row_offsets)[expect != UNREACHED].sum())
 
 
 def test_device_info(dfq):
@@ -374,6 +376,7 @@ def test_device_seq_one_launch(dfq, oracle, gname, direction_opt):
             rep = csr.validate(s, levels[i], parents[i])
             assert rep["errors"] == 0, (i, s, rep)
             assert st.reached == int((e != UNREACHED).sum())
+            assert st.edges == int(deg[e != UNREACHED].sum()), (i, s)
             assert st.depth == int(e[e != UNREACHED].max()) + 1
             assert st.kernel_ms > 0
         assert sum(st.kernel_ms for st in stats) <= launch_ms * 1.05 + 0.05
@@ -384,6 +387,7 @@ def test_device_seq_one_launch(dfq, oracle, gname, direction_opt):
         # a plain launch afterwards shares no state with the sequence
         r = g.bfs(srcs[1])
         np.testing.assert_array_equal(r.level, levels[1])
+        assert r.edges == stats[1].edges and r.reached == stats[1].reached
 
 
 def test_device_seq_bad_input(dfq):

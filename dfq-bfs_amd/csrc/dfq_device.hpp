@@ -67,8 +67,8 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
   return x;
 }
 
-// Sum over the 64 lanes of a value below 2^40 (the barrier's sums are
-// 40-bit fields), in a scalar register: DPP scans of its two 20-bit
+// Sum over the 64 lanes of a value below 2^40 (claim counts and out-degree
+// sums are below 2^40), in a scalar register: DPP scans of its two 20-bit
 // halves (neither half's sum can overflow 32 bits), no LDS permutes.
 // Every lane of the wave must be active.
 __device__ __forceinline__ uint64_t wave_sum_u40(uint64_t v) {
@@ -122,14 +122,18 @@ __device__ __forceinline__ void tails_prefix(const uint32_t *tails, uint32_t *pr
 // Workgroups find their XCD from the XCC_ID hardware register; the number
 // of workgroups per XCD is counted at kernel start (barrier_census), because
 // the dispatcher's block -> XCD rotation changes from launch to launch.
-// Arrival words pack count (bits 40..63) and payload (bits 0..39).
+// Arrival words pack count (bits 54..63) and payload (bits 0..53); the
+// release words generation (bits 54..63, modulo 1024) and payload sum.
 // ---------------------------------------------------------------------------
 constexpr int kMaxXcd = 16;
-constexpr uint64_t kArrive = 1ull << 40;
+constexpr int kCountShift = 54;
+constexpr uint64_t kArrive = 1ull << kCountShift;
 constexpr uint64_t kPayloadMask = kArrive - 1;
-// Largest CSR entry count a search accepts: the barrier's payload and side
-// sums (claims, out-degree sums) are 40-bit fields.
-constexpr uint64_t kMaxEntries = kPayloadMask;
+constexpr uint32_t kGenMask = (1u << (64 - kCountShift)) - 1;
+// Largest CSR entry count a search accepts: claim counts and out-degree
+// sums travel as sums below 2^40 (wave and workgroup sums, barrier
+// payloads).
+constexpr uint64_t kMaxEntries = (1ull << 40) - 1;
 
 struct alignas(256) BarrierWord {
   uint64_t word;
@@ -138,11 +142,11 @@ struct alignas(256) BarrierWord {
 
 struct GridBarrier {
   BarrierWord xleaf[kMaxXcd];   // per-XCD arrivals
-  BarrierWord xside[kMaxXcd];   // per-XCD side arrivals: count (bits 40..63) | side sum
+  BarrierWord xside[kMaxXcd];   // per-XCD side arrivals: count (bits 54..63) | side sum
   BarrierWord root;             // XCD-leader arrivals
   BarrierWord root_side;        // XCD side arrivals: count | side sum
-  BarrierWord side_out;         // generation (bits 40..63) | side total of that round
-  BarrierWord gensum;           // generation (bits 40..63) | payload sum (bits 0..39)
+  BarrierWord side_out;         // generation (bits 54..63) | side total of that round
+  BarrierWord gensum;           // generation (bits 54..63) | payload sum (bits 0..53)
   BarrierWord census_epoch;     // launches counted so far (low bit picks the census set)
   BarrierWord ticket;           // workgroup ranks handed out within a team
   uint32_t census[2][kMaxXcd];  // workgroups per XCD, one set per launch parity
@@ -222,7 +226,7 @@ __device__ __forceinline__ void barrier_census(GridBarrier *b, TeamBar team_bar,
   uint32_t c = 0, g = 0;
   for (;;) {
     if (lane < (uint32_t)kMaxXcd) c = ld_relaxed(&cnt[lane]);
-    else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> 40);
+    else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> kCountShift);
     uint32_t sum = c;
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
@@ -306,7 +310,7 @@ __device__ __forceinline__ bool plain_census(GridBarrier *b, uint32_t epoch, uin
   // the grid is complete, so the per-XCD counts are final
   uint32_t c = 0, g = 0;
   if (lane < (uint32_t)kMaxXcd) c = ld_relaxed(&cnt[lane]);
-  else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> 40);
+  else if (lane < 2u * kMaxXcd) g = (uint32_t)(ld_relaxed(&team_bar(lane - kMaxXcd)->gensum.word) >> kCountShift);
   const uint32_t nz = (uint32_t)__ballot(lane < (uint32_t)kMaxXcd && c != 0);
   const uint32_t below = nz & ((1u << (lane & 31u)) - 1u);
   if (lane < (uint32_t)kMaxXcd && c != 0) count[__popc(below)] = c;
@@ -350,7 +354,7 @@ __device__ __forceinline__ void team_ticket_reset(GridBarrier *tb) {
 }
 
 // All threads of every workgroup must call it.  `payload` and `side` (each
-// summing to < 2^40 over the grid) are read from thread 0; returns the
+// summing to < 2^54 over the grid) are read from thread 0; returns the
 // grid-wide payload sum in every thread and, when side_total is given (by
 // every workgroup alike), the side sum there.  `st` is only meaningful in
 // thread 0.  `s_word` points at two 64-bit LDS slots.
@@ -379,8 +383,8 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
   wait_own_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t next = (st.gen + 1) & 0xffffffu;
-    const uint64_t tag = (uint64_t)next << 40;
+    const uint32_t next = (st.gen + 1) & kGenMask;
+    const uint64_t tag = (uint64_t)next << kCountShift;
     // both arrivals in flight together
     uint64_t os = 0;
     if (with_side)
@@ -392,8 +396,8 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
     // sum; on the payload word it writes the L2 back (every workgroup of the
     // XCD has its stores there) and forwards the payload sum.  One
     // workgroup may be both; its two root arrivals then go out together.
-    const bool lead_s = with_side && (os >> 40) + 1 == st.xcc_n;
-    const bool lead_p = (old >> 40) + 1 == st.xcc_n;
+    const bool lead_s = with_side && (os >> kCountShift) + 1 == st.xcc_n;
+    const bool lead_p = (old >> kCountShift) + 1 == st.xcc_n;
     const uint64_t xs = (os & kPayloadMask) + side;
     const uint64_t xsum = (old & kPayloadMask) + payload;
     if (lead_s) __hip_atomic_store(&b->xside[st.xcc].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -410,23 +414,29 @@ __device__ __forceinline__ uint64_t grid_barrier(GridBarrier *b, BarrierState &s
       rs = __hip_atomic_fetch_add(&b->root_side.word, kArrive | xs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (lead_p)
       rp = __hip_atomic_fetch_add(&b->root.word, kArrive | xsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const bool fin_s = lead_s && (rs >> 40) + 1 == st.nx;
-    const bool fin_p = lead_p && (rp >> 40) + 1 == st.nx;
+    const bool fin_s = lead_s && (rs >> kCountShift) + 1 == st.nx;
+    const bool fin_p = lead_p && (rp >> kCountShift) + 1 == st.nx;
     if (fin_s) __hip_atomic_store(&b->root_side.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fin_p) __hip_atomic_store(&b->root.word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fin_s || fin_p) wait_own_stores();
     if (fin_s)
       __hip_atomic_store(&b->side_out.word, tag | ((rs & kPayloadMask) + xs), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-    if (fin_p)
+    if (fin_p) {
       __hip_atomic_store(&b->gensum.word, tag | ((rp & kPayloadMask) + xsum), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
+      // generations repeat every 1024 barriers: keep the side word's current
+      // too, so that a later barrier with a side sum cannot take a word
+      // left over from 1024 barriers without one for its own
+      if (!with_side)
+        __hip_atomic_store(&b->side_out.word, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // every workgroup waits on the root's words themselves (no relay)
     uint64_t w, ws = tag;
     for (;;) {
       w = ld_relaxed(&b->gensum.word);
       if (with_side) ws = ld_relaxed(&b->side_out.word);
-      if ((w >> 40) == next && (ws >> 40) == next) break;
+      if ((w >> kCountShift) == next && (ws >> kCountShift) == next) break;
       __builtin_amdgcn_s_sleep(1);
     }
     // this CU's L1 may hold stale lines; on gfx950 only the device-scope

@@ -440,6 +440,7 @@ __global__ __launch_bounds__(BLOCK) void dfq_batch_kernel(BatchArgs b) {
     a.n = gr.n;
     a.n_words = (gr.n + 31) / 32;
     a.m = gr.m;
+    a.fuse_bits = fuse_bits_for(gr.n, gr.m);
     a.iso = gr.iso_base != kNoIso ? b.iso + gr.iso_base : nullptr;
     a.source = q.source;
     a.small_max = q.small_max;

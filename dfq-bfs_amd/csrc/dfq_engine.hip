@@ -162,19 +162,6 @@ const void *seq_kernel_for(uint32_t block, Variant v = Variant::kAny) {
   }
 }
 
-// Bits of the level barrier's payload that carry the claim count when the
-// out-degree sum shares the word (claims < 2^bits(N), sums <= M), or 0
-// when the two do not fit its 54 bits together.
-uint32_t fuse_bits(uint64_t n, uint64_t m) {
-  auto bits = [](uint64_t x) {
-    uint32_t b = 0;
-    while (x >> b) ++b;
-    return b;
-  };
-  const uint32_t bn = bits(n);
-  return bn + bits(m) <= (uint32_t)kCountShift ? bn : 0u;
-}
-
 // The instantiation a search with these arguments runs.
 Variant variant_for(const dfq_graph *g, const BfsArgs &a) {
   if (a.bottom_up) return Variant::kAny;
@@ -280,7 +267,7 @@ BfsArgs make_args(dfq_graph *g, uint32_t source, uint32_t *d_level, uint32_t *d_
   a.source = source;
   a.small_max = g->cfg.small_max;
   a.deg_max = (uint32_t)std::min<uint64_t>(g->max_deg, 0xffffffffu);
-  a.fuse_bits = fuse_bits(g->n, g->m);
+  a.fuse_bits = fuse_bits_for(g->n, g->m);
   a.medium_max = g->cfg.medium_max;
   a.large_max = g->cfg.large_max;
   a.bottom_up = (g->cfg.direction_opt && g->symmetric) ? 1u : 0u;

@@ -340,6 +340,16 @@ __device__ __forceinline__ uint64_t block_sum_mf(SharedAll<BLOCK> &sh, V &vis) {
   return t;
 }
 
+// Bits of the level barrier's payload that carry the claim count when the
+// out-degree sum shares the word (claims < 2^bits(N), sums <= M), or 0
+// when the two do not fit its kCountShift bits together.
+__host__ __device__ inline uint32_t fuse_bits_for(uint64_t n, uint64_t m) {
+  uint32_t bn = 0, bm = 0;
+  while (bn < 64 && (n >> bn)) ++bn;
+  while (bm < 64 && (m >> bm)) ++bm;
+  return bn + bm <= (uint32_t)kCountShift ? bn : 0u;
+}
+
 // Level barrier: returns the team's claim count and sets *mf_total to the
 // out-degree sum of the claimed vertices.  With a.fuse_bits both travel in
 // the one payload word (claims below 2^fuse_bits, the sum above them; the

@@ -404,6 +404,8 @@ __global__ __launch_bounds__(BLOCK) void dfq_serve_kernel(ServeArgs s) {
     a.alpha = d.alpha;
     a.bu_exit = d.beta ? g.n / d.beta : 0;
     a.m = g.m;
+    a.deg_max = 0;
+    a.fuse_bits = fuse_bits_for(g.n, g.m);
     // the system-scope barrier right after writes every L2 back
     grid_bfs<BLOCK>(a, SearchIo{a.level, a.parent, a.result, a.source}, sh, team, GridBfsMode{false, true, 0});
     // outputs reach memory the host's copies read before the result appears

@@ -286,10 +286,12 @@ __device__ __forceinline__ void expand_large(V &vis, SharedAll<BLOCK> &sh, uint3
 
 // Moves every wave's staged entries to the segment with one atomic per
 // workgroup, and sums the wave-uniform `v` over the workgroup (thread 0
-// gets the total in the return value).  Must be reached by all threads of
-// the workgroup.
+// gets the total in the return value).  With `mf` it also sums the lanes'
+// out-degree sums of claimed vertices into *mf (every thread) and resets
+// them, in the same pass.  Must be reached by all threads of the workgroup.
 template <int BLOCK, class V>
-__device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, uint32_t wave, uint32_t v = 0) {
+__device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, uint32_t wave, uint32_t v = 0,
+                                                uint64_t *mf = nullptr) {
   constexpr int W = KernelShape<BLOCK>::kWaves;
   if (lane_id() == 0) {
     sh.stage_n[wave] = vis.out_n;
@@ -312,7 +314,18 @@ __device__ __forceinline__ uint32_t flush_block(V &vis, SharedAll<BLOCK> &sh, ui
   const uint32_t cnt = vis.out_n, base = sh.stage_base[wave];
   for (uint32_t k = lane_id(); k < cnt; k += kWave) vis.seg[base + k] = vis.stage[k];
   vis.out_n = 0;
+  if (mf) {
+    const uint64_t w = wave_sum_u40(vis.mf);
+    vis.mf = 0;
+    if (lane_id() == 0) sh.wave_mf[wave] = w;
+  }
   __syncthreads();
+  if (mf) {
+    // the slots are written again only after two more __syncthreads
+    uint64_t t = 0;
+    for (int i = 0; i < W; ++i) t += sh.wave_mf[i];
+    *mf = uniform(t);
+  }
   return sum;
 }
 
@@ -1162,8 +1175,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         }
         topdown_phase_a<BLOCK, kProf, kLow>(a, sh, vis, in_seg, in_slot, cur, wave, ra, solo ? 1u : tsize, edges);
       }
-      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
-      const uint64_t wg_mf = kLow ? 0 : block_sum_mf<BLOCK>(sh, vis);
+      uint64_t wg_mf = 0;
+      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims, kLow ? nullptr : &wg_mf);
       if (kProf && a.prof && tid == 0) busy += now_ns() - t_start;
       uint64_t mf1 = 0;
       if (solo) {
@@ -1186,8 +1199,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
         // a workgroup reads work items only when its first wave has one
         fresh_bit(fb_zero, (solo ? 0u : trank * (uint32_t)S::kWaves) < hc);
         topdown_phase_h<BLOCK, kTd>(a, sh, vis, hc, r_gwave, r_nwaves, edges);
-        const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before);
-        const uint64_t wg_mf2 = block_sum_mf<BLOCK>(sh, vis);
+        uint64_t wg_mf2 = 0;
+        const uint32_t wg2 = flush_block<BLOCK>(vis, sh, wave, vis.claims - before, &wg_mf2);
         if (kProf && a.prof && tid == 0) busy += now_ns() - th;
         uint64_t mf2 = 0;
         if (solo) {
@@ -1215,8 +1228,8 @@ __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, S
       uint64_t tb = 0;
       if (kProf && a.prof && tid == 0) tb = now_ns();
       bottomup_level<BLOCK, kProf>(a, io, sh, vis, L, fb_cur, fb_nxt, wave, gwave, nwaves, edges);
-      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims);
-      const uint64_t wg_mf = block_sum_mf<BLOCK>(sh, vis);
+      uint64_t wg_mf = 0;
+      const uint32_t wg_claims = flush_block<BLOCK>(vis, sh, wave, vis.claims, &wg_mf);
       if (kProf && a.prof && tid == 0) busy += now_ns() - tb;
       uint64_t mf1 = 0;
       n_next = level_barrier(a, bst, wg_claims, wg_mf, sh.bcast, &mf1);

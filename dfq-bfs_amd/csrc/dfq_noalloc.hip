@@ -182,6 +182,7 @@ extern "C" int dfq_bfs_device_noalloc(const int64_t *d_row_offsets, const uint32
   a.alpha = 0;
   a.bu_exit = 0;
   a.m = m;
+  a.fuse_bits = fuse_bits_for(n, m);
   // one plain launch whose grid assembles itself; cooperative when it does
   // not (other work held CUs), as dfq_bfs_device does
   Control c;

@@ -1000,8 +1000,8 @@ struct GridBfsMode {
 // kTd: every level top-down (a.bottom_up is ignored), compiled without the
 // bottom-up code; the work items of a snapshot level with more than
 // kLiveProbeRatio out-edges per visited vertex also probe the live bitmap.
-// That probe stays out of the kernels that may go bottom-up: the symmetric
-// graphs they serve reach snapshot levels in top-down mode only, and its
+// That probe stays out of the kernels that may go bottom-up: their large
+// levels run bottom-up, so they seldom take a snapshot, and the probe's
 // code alone cost the hybrid RMAT-22 search ~1 us.
 template <int BLOCK, bool kProf = false, bool kTd = false, bool kLow = false>
 __device__ __forceinline__ void grid_bfs(const BfsArgs &a, const SearchIo &io, SharedAll<BLOCK> &sh, Team &team,

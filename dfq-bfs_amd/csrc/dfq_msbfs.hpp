@@ -625,8 +625,9 @@ __device__ __forceinline__ void ms_grid_search(const MsArgs &a, SharedAll<BLOCK>
     if (tid == 0 && ms_or)
       __hip_atomic_fetch_or(&in_slot.active, ms_or, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     vis.mf = mf;
-    uint64_t m_f = 0;
-    grid_barrier(&ctl->bar, bst, 0, sh.bcast, block_sum_mf<BLOCK>(sh, vis), &m_f);
+    // the out-degree sum is this barrier's only sum: it rides as the
+    // payload, with no second chain of arrivals
+    const uint64_t m_f = grid_barrier(&ctl->bar, bst, block_sum_mf<BLOCK>(sh, vis), sh.bcast);
     const uint64_t alive = uniform(ld_relaxed(&in_slot.active));
     if (trank == 0 && tid < a.count && ((alive >> tid) & 1ull)) my_depth = L + 1;
     uint64_t t_e = 0;
